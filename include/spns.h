@@ -1,0 +1,160 @@
+/*
+ * spns.h -- C ABI of the MI355X pseudospectral HIT step.
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success or a
+ * SPNS_E* code; spns_last_error() gives the message.  Calls on one context
+ * are ordered on that context's HIP stream; use one host thread per context.
+ *
+ * The reference (adalbal/SpNS) has no FFI: its seam is the C++ class surface
+ * `main` uses.  Each entry point below names the reference member it replaces.
+ * The C++ facade spns_amd/csrc/hit.hpp re-exposes these under the
+ * reference's own class and method names.
+ *
+ * Spectral state layout ("compact box", the reference's Fourier-dump order,
+ * src/HIT_LES_output.cpp:46-53 with one rank): component-major
+ * [c][i2][i1][i3] complex double (re, im), with
+ *   k2 = i2 <= Ny/2 ? i2 : i2 - (2*(Ny/2)+1),  i2 in [0, 2*(Ny/2)+1)
+ *   k1 = i1 <= Nx/2 ? i1 : i1 - (2*(Nx/2)+1),  i1 in [0, 2*(Nx/2)+1)
+ *   k3 = i3,                                    i3 in [0, Nz/2]
+ * i.e. the retained box |k1|<=Nx/2, |k2|<=Ny/2, 0<=k3<=Nz/2 of
+ * src/HIT_LES.cpp:860 (modes masked by length factors are present and zero).
+ */
+#ifndef SPNS_H
+#define SPNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SPNS_OK = 0,
+    SPNS_EINVAL = 1,       /* bad argument */
+    SPNS_EUNSUPPORTED = 2, /* grid length not compiled in, or feature off */
+    SPNS_ENOMEM = 3,       /* device allocation failed */
+    SPNS_EHIP = 4,         /* HIP runtime error */
+    SPNS_ENONFINITE = 5    /* the CFL reduction saw a NaN/Inf velocity */
+};
+
+enum { SPNS_PREC_FP64 = 0, SPNS_PREC_FP32_SPLIT = 1 };
+
+typedef struct spns_ctx spns_ctx;
+
+/* Arguments of HIT::HIT (src/HIT_LES.h:23-29) after the L-factor expansion
+ * of src/MAIN.cpp:216-253, plus the build's options. */
+typedef struct spns_config {
+    int32_t Nx, Ny, Nz;      /* retained modes (src/HIT_LES.h:23) */
+    int32_t Mx, My, Mz;      /* FFT grid; 0 => (3N)/2 (src/MAIN.cpp:177) */
+    double nu, C_Smag, C_At; /* C_Smag > 0 turns LES on (src/MAIN.cpp:272) */
+    int32_t Lx, Ly, Lz;      /* length factors as HIT receives them */
+    double omega_x, omega_y, omega_z;
+    int32_t isNotCubic, isSelfAdaptive, isComplexConjugateCorrected;
+    int32_t precision;       /* SPNS_PREC_FP64 | SPNS_PREC_FP32_SPLIT */
+    double delta;            /* LES filter width; <= 0 => 2pi/cbrt(...) (src/HIT_LES.h:53) */
+    double fixed_dt;         /* > 0 => fixed time step instead of the CFL step */
+    int32_t force_on;        /* constant-power forcing (src/HIT_LES.cpp:243-278) */
+    double force_eps;        /* injection rate */
+    int32_t force_kmin, force_kmax;
+    int32_t device;          /* HIP device ordinal */
+    int32_t use_graph;       /* capture the step as a hipGraph */
+    int32_t reserved[6];
+} spns_config;
+
+typedef struct spns_scalars {
+    double time;     /* HIT::gettime */
+    double dt;       /* HIT::getAt */
+    double nu;       /* HIT::getnu */
+    double maxvel;   /* CFL velocity of the last antitransformed field */
+    double ek;       /* HIT::getEk_Tot (E of the current state) */
+    double om;       /* HIT::getOk_Tot (enstrophy; eps = nu*om) */
+    double relambda; /* HIT::getReLambda */
+    double skewness; /* <(du/dx)^3>/<(du/dx)^2>^1.5 at the start of the last step */
+    double kappa;    /* self-adaptive scheme's kappa */
+    int64_t iter;    /* steps taken */
+} spns_scalars;
+
+/* Replaces HIT::HIT + HIT_init allocation (src/HIT_LES.h:23-84, src/HIT_LES.cpp:770-1092). */
+int spns_create(const spns_config *cfg, spns_ctx **out);
+/* Replaces HIT::~HIT / HIT_destroy (src/HIT_LES.cpp:1094-1130). */
+int spns_destroy(spns_ctx *ctx);
+/* Number of compact modes per component, and of shells K=0..last_rad_max. */
+int64_t spns_nmodes(const spns_ctx *ctx);
+int32_t spns_nbins(const spns_ctx *ctx);
+/* last_rad (src/HIT_LES.h:48), the K41 / forcing shell limit. */
+int32_t spns_last_rad(const spns_ctx *ctx);
+/* Human-readable message of the last failing call. */
+const char *spns_last_error(const spns_ctx *ctx);
+
+/* Initial fields.  Each leaves R = 0, time = 0, iter = 0; then call
+ * spns_finish_init (src/HIT_LES.cpp:1037-1086). */
+/* State in the compact layout above: 3*nmodes complex (6*nmodes doubles). */
+int spns_set_state(spns_ctx *ctx, const double *state);
+int spns_get_state(spns_ctx *ctx, double *state);
+/* R history (R of the last step), compact layout. */
+int spns_get_rhist(spns_ctx *ctx, double *rhist);
+/* HIT::Input_Real_Field (src/HIT_LES.cpp:542-599, 738-763): u,v,w are
+ * [x][y][z] of nxf*nyf*nzf doubles; ek_file/ens_file receive
+ * Ek_init_file / Enstrophy_init_file (src/HIT_LES.cpp:1268-1294). */
+int spns_init_from_physical(spns_ctx *ctx, const double *u, const double *v, const double *w,
+                            int32_t nxf, int32_t nyf, int32_t nzf, double *ek_file, double *ens_file);
+/* HIT::Input_K41_Field (src/HIT_LES.cpp:502-540); nranks > 1 reproduces the
+ * per-rank srand(0)/rand() draws of an nranks-slab MPI run. */
+int spns_init_k41(spns_ctx *ctx, int32_t nranks);
+/* Seeded random-phase solenoidal field with E(k) ~ k^4 exp(-2 (k/k0)^2),
+ * scaled to u_rms (no reference counterpart: synthetic workload input). */
+int spns_init_random(spns_ctx *ctx, uint64_t seed, double k0, double urms);
+/* Rest of HIT_init: R=0, antitransform, CFL step unless isReLambda, Ek_init. */
+int spns_finish_init(spns_ctx *ctx, int32_t isReLambda);
+
+/* HIT::New_Fractional_Step_Method (src/HIT_LES.cpp:9-25), nsteps times. */
+int spns_step(spns_ctx *ctx, int32_t nsteps);
+/* HIT::Recalculate_TimeStep (src/HIT_LES.cpp:365-371). */
+int spns_recalculate_timestep(spns_ctx *ctx);
+/* HIT::setnu (src/HIT_LES.h:282). */
+int spns_set_nu(spns_ctx *ctx, double nu);
+/* HIT::Forcing_Reynolds_Lambda (src/HIT_LES.cpp:485-489). */
+int spns_forcing_reynolds_lambda(spns_ctx *ctx, double relambda);
+/* HIT::Recalculate_Reynolds_Lambda (src/HIT_LES.cpp:478-483). */
+int spns_recalculate_reynolds_lambda(spns_ctx *ctx, double *relambda);
+/* HIT::Forcing_Energy_Cascade (src/HIT_LES.cpp:410-444); forced_ek has
+ * last_input_rad+1 entries. */
+int spns_forcing_energy_cascade(spns_ctx *ctx, const double *forced_ek, int64_t last_input_rad, int32_t nullify);
+/* HIT::Complex_Conjugate_Correction (src/HIT_LES.cpp:446-476). */
+int spns_complex_conjugate_correction(spns_ctx *ctx);
+/* HIT::Recalculate_Energy_Cascade (src/HIT_LES.cpp:1261-1266): E(K) for
+ * K = 0..nbins-1 into ek (may be NULL), total into ek_tot (may be NULL). */
+int spns_energy_cascade(spns_ctx *ctx, double *ek, int32_t nbins, double *ek_tot);
+/* HIT getters (src/HIT_LES.h:187-201, 277-291). */
+int spns_get_scalars(spns_ctx *ctx, spns_scalars *out);
+/* HIT::Recalculate_Velocity_Antitransform + RealVelocity_to_BinaryFile's
+ * data (src/HIT_LES.cpp:60-69): u,v,w as [3][Mx][My][Mz] doubles. */
+int spns_velocity_physical(spns_ctx *ctx, double *uvw);
+/* Wait for the context's stream. */
+int spns_synchronize(spns_ctx *ctx);
+/* Device time (ms) of the last spns_step call's kernels, from HIP events. */
+int spns_last_step_ms(spns_ctx *ctx, double *ms);
+
+/* ---- FFT_package drop-in (src/FFTW_package.h:21-71) on device buffers ----
+ * complex layout [y][x][Mz/2+1] (FFTW_MPI_TRANSPOSED, one rank),
+ * real layout [x][y][2(Mz/2+1)] (src/util.h:75-80).  Both transforms are
+ * unnormalised and destroy their input, like the FFTW plans they replace. */
+typedef struct spns_fft spns_fft;
+/* FFT_package::FFT_package / FFT_init (src/FFTW_package.cpp:47-56). */
+int spns_fft_create(int32_t Mx, int32_t My, int32_t Mz, int32_t precision, int32_t device, spns_fft **out);
+int spns_fft_destroy(spns_fft *fft);
+/* FFT_package::FFTWr2c (src/FFTW_package.cpp:7-9). */
+int spns_fft_r2c(spns_fft *fft, void *rin, void *cout);
+/* FFT_package::IFFTWc2r (src/FFTW_package.cpp:12-14). */
+int spns_fft_c2r(spns_fft *fft, void *cin, void *rout);
+/* FFT_package::Fourier_Normalization(COMPLEX*, relevant) (src/FFTW_package.cpp:33-42)
+ * with relevant = the retained box of (Nx,Ny,Nz); Nx<=0 normalises all. */
+int spns_fft_normalize(spns_fft *fft, void *c, int32_t Nx, int32_t Ny, int32_t Nz);
+int spns_fft_synchronize(spns_fft *fft);
+const char *spns_fft_last_error(const spns_fft *fft);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPNS_H */

@@ -1,0 +1,305 @@
+"""MI355X-native pseudospectral HIT step (drop-in for SpNS's HIT time step).
+
+Python binding of the C ABI in include/spns.h (libspns.so, built in-tree by
+`python -m spns_amd.build`).  There is no CPU fallback: if the HIP library
+is missing or no GPU is present, constructing a Solver raises.
+
+    s = Solver(Nx=32, nu=0.01, fixed_dt=1e-3)
+    s.init_random(seed=1, k0=4.0)
+    s.finish_init()
+    s.step(200)
+    print(s.scalars(), s.spectrum())
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspns.so")
+PREC_FP64 = 0
+PREC_FP32_SPLIT = 1
+_lib = None
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32), ("Nz", ctypes.c_int32),
+        ("Mx", ctypes.c_int32), ("My", ctypes.c_int32), ("Mz", ctypes.c_int32),
+        ("nu", ctypes.c_double), ("C_Smag", ctypes.c_double), ("C_At", ctypes.c_double),
+        ("Lx", ctypes.c_int32), ("Ly", ctypes.c_int32), ("Lz", ctypes.c_int32),
+        ("omega_x", ctypes.c_double), ("omega_y", ctypes.c_double), ("omega_z", ctypes.c_double),
+        ("isNotCubic", ctypes.c_int32), ("isSelfAdaptive", ctypes.c_int32),
+        ("isComplexConjugateCorrected", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("delta", ctypes.c_double), ("fixed_dt", ctypes.c_double),
+        ("force_on", ctypes.c_int32), ("force_eps", ctypes.c_double),
+        ("force_kmin", ctypes.c_int32), ("force_kmax", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
+    ]
+
+
+class Scalars(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("nu", ctypes.c_double),
+                ("maxvel", ctypes.c_double), ("ek", ctypes.c_double), ("om", ctypes.c_double),
+                ("relambda", ctypes.c_double), ("skewness", ctypes.c_double), ("kappa", ctypes.c_double),
+                ("iter", ctypes.c_int64)]
+
+
+EXPORTS = [
+    "spns_create", "spns_destroy", "spns_nmodes", "spns_nbins", "spns_last_rad", "spns_last_error",
+    "spns_set_state", "spns_get_state", "spns_get_rhist", "spns_init_from_physical", "spns_init_k41",
+    "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
+    "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
+    "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
+    "spns_velocity_physical", "spns_synchronize", "spns_last_step_ms",
+    "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
+    "spns_fft_synchronize", "spns_fft_last_error",
+]
+
+
+def load_library(path=LIB_PATH):
+    """Loads libspns.so and declares the C ABI.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("spns_amd: %s is not built; run `python -m spns_amd.build`" % path)
+    L = ctypes.CDLL(path)
+    P, D, I32, I64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32, ctypes.c_int64
+    sig = {
+        "spns_create": (I32, [ctypes.POINTER(Config), ctypes.POINTER(P)]),
+        "spns_destroy": (I32, [P]),
+        "spns_nmodes": (I64, [P]),
+        "spns_nbins": (I32, [P]),
+        "spns_last_rad": (I32, [P]),
+        "spns_last_error": (ctypes.c_char_p, [P]),
+        "spns_set_state": (I32, [P, D]),
+        "spns_get_state": (I32, [P, D]),
+        "spns_get_rhist": (I32, [P, D]),
+        "spns_init_from_physical": (I32, [P, D, D, D, I32, I32, I32, D, D]),
+        "spns_init_k41": (I32, [P, I32]),
+        "spns_init_random": (I32, [P, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
+        "spns_finish_init": (I32, [P, I32]),
+        "spns_step": (I32, [P, I32]),
+        "spns_recalculate_timestep": (I32, [P]),
+        "spns_set_nu": (I32, [P, ctypes.c_double]),
+        "spns_forcing_reynolds_lambda": (I32, [P, ctypes.c_double]),
+        "spns_recalculate_reynolds_lambda": (I32, [P, D]),
+        "spns_forcing_energy_cascade": (I32, [P, D, I64, I32]),
+        "spns_complex_conjugate_correction": (I32, [P]),
+        "spns_energy_cascade": (I32, [P, D, I32, D]),
+        "spns_get_scalars": (I32, [P, ctypes.POINTER(Scalars)]),
+        "spns_velocity_physical": (I32, [P, D]),
+        "spns_synchronize": (I32, [P]),
+        "spns_last_step_ms": (I32, [P, D]),
+        "spns_fft_create": (I32, [I32, I32, I32, I32, I32, ctypes.POINTER(P)]),
+        "spns_fft_destroy": (I32, [P]),
+        "spns_fft_r2c": (I32, [P, P, P]),
+        "spns_fft_c2r": (I32, [P, P, P]),
+        "spns_fft_normalize": (I32, [P, P, I32, I32, I32]),
+        "spns_fft_synchronize": (I32, [P]),
+        "spns_fft_last_error": (ctypes.c_char_p, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class SpnsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("spns error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Solver:
+    """One HIT run on the GPU.  Keyword arguments follow HIT::HIT
+    (src/HIT_LES.h:23-29) after MAIN's length-factor expansion, plus
+    precision, delta, fixed_dt and constant-power forcing options."""
+
+    def __init__(self, Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag=0.0, C_At=1.0,
+                 Lx=1, Ly=1, Lz=1, omega=(0.0, 0.0, 0.0), isNotCubic=False, isSelfAdaptive=False,
+                 isCCC=False, precision=PREC_FP64, delta=0.0, fixed_dt=0.0, force_eps=None,
+                 force_kmin=1, force_kmax=2, device=0, use_graph=True):
+        self.L = load_library()
+        Ny = Nx if Ny is None else Ny
+        Nz = Nx if Nz is None else Nz
+        c = Config()
+        c.Nx, c.Ny, c.Nz = Nx, Ny, Nz
+        c.Mx, c.My, c.Mz = Mx or 0, My or 0, Mz or 0
+        c.nu, c.C_Smag, c.C_At = nu, C_Smag, C_At
+        c.Lx, c.Ly, c.Lz = Lx, Ly, Lz
+        c.omega_x, c.omega_y, c.omega_z = omega
+        c.isNotCubic, c.isSelfAdaptive, c.isComplexConjugateCorrected = int(isNotCubic), int(isSelfAdaptive), int(isCCC)
+        c.precision = precision
+        c.delta, c.fixed_dt = delta, fixed_dt
+        c.force_on = int(force_eps is not None)
+        c.force_eps = 0.0 if force_eps is None else force_eps
+        c.force_kmin, c.force_kmax = force_kmin, force_kmax
+        c.device = device
+        c.use_graph = int(use_graph)
+        self.cfg = c
+        h = ctypes.c_void_p()
+        rc = self.L.spns_create(ctypes.byref(c), ctypes.byref(h))
+        if rc:
+            raise SpnsError(rc, self.L.spns_last_error(None).decode())
+        self.h = h
+        self.nmodes = self.L.spns_nmodes(h)
+        self.nbins = self.L.spns_nbins(h)
+        self.last_rad_max = self.nbins - 1
+        self.last_rad = self.L.spns_last_rad(h)
+        self.shape = ((3 * Nx) // 2 if not Mx else Mx, (3 * Ny) // 2 if not My else My, (3 * Nz) // 2 if not Mz else Mz)
+        self.nk = (2 * (Ny // 2) + 1, 2 * (Nx // 2) + 1, Nz // 2 + 1)
+
+    def _chk(self, rc):
+        if rc:
+            raise SpnsError(rc, self.L.spns_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.spns_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- initial fields (HIT_init) ---
+    def set_state(self, state):
+        s = np.ascontiguousarray(state, dtype=np.complex128).reshape(3, self.nmodes)
+        self._chk(self.L.spns_set_state(self.h, _dp(s.view(np.float64))))
+
+    def get_state(self):
+        out = np.zeros((3, self.nmodes), dtype=np.complex128)
+        self._chk(self.L.spns_get_state(self.h, _dp(out.view(np.float64))))
+        return out
+
+    def get_rhist(self):
+        out = np.zeros((3, self.nmodes), dtype=np.complex128)
+        self._chk(self.L.spns_get_rhist(self.h, _dp(out.view(np.float64))))
+        return out
+
+    def init_from_physical(self, u, v, w):
+        nf = u.shape
+        u, v, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w))
+        ekf, ensf = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.L.spns_init_from_physical(self.h, _dp(u), _dp(v), _dp(w), nf[0], nf[1], nf[2],
+                                                 ctypes.byref(ekf), ctypes.byref(ensf)))
+        return ekf.value, ensf.value
+
+    def init_k41(self, nranks=4):
+        self._chk(self.L.spns_init_k41(self.h, nranks))
+
+    def init_random(self, seed=0, k0=4.0, urms=1.0):
+        self._chk(self.L.spns_init_random(self.h, seed, k0, urms))
+
+    def finish_init(self, isReLambda=False):
+        self._chk(self.L.spns_finish_init(self.h, int(isReLambda)))
+
+    # --- stepping / forcing ---
+    def step(self, n=1):
+        self._chk(self.L.spns_step(self.h, n))
+
+    def synchronize(self):
+        self._chk(self.L.spns_synchronize(self.h))
+
+    def last_step_ms(self):
+        ms = ctypes.c_double()
+        self._chk(self.L.spns_last_step_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def recalc_timestep(self):
+        self._chk(self.L.spns_recalculate_timestep(self.h))
+
+    def set_nu(self, nu):
+        self._chk(self.L.spns_set_nu(self.h, nu))
+
+    def forcing_relambda(self, relambda):
+        self._chk(self.L.spns_forcing_reynolds_lambda(self.h, relambda))
+
+    def relambda(self):
+        r = ctypes.c_double()
+        self._chk(self.L.spns_recalculate_reynolds_lambda(self.h, ctypes.byref(r)))
+        return r.value
+
+    def forcing_energy_cascade(self, ek, last_input_rad, nullify=False):
+        ek = np.ascontiguousarray(ek, dtype=np.float64)
+        self._chk(self.L.spns_forcing_energy_cascade(self.h, _dp(ek), last_input_rad, int(nullify)))
+
+    def ccc(self):
+        self._chk(self.L.spns_complex_conjugate_correction(self.h))
+
+    # --- diagnostics ---
+    def spectrum(self):
+        ek = np.zeros(self.nbins)
+        tot = ctypes.c_double()
+        self._chk(self.L.spns_energy_cascade(self.h, _dp(ek), self.nbins, ctypes.byref(tot)))
+        return ek
+
+    def energy(self):
+        return self.scalars()["ek"]
+
+    def enstrophy(self):
+        return self.scalars()["om"]
+
+    def scalars(self):
+        s = Scalars()
+        self._chk(self.L.spns_get_scalars(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Scalars._fields_}
+
+    def velocity_physical(self):
+        Mx, My, Mz = self.shape
+        out = np.zeros((3, Mx, My, Mz))
+        self._chk(self.L.spns_velocity_physical(self.h, _dp(out)))
+        return out
+
+
+class Fft:
+    """FFT_package drop-in on device buffers (src/FFTW_package.h:21-71).
+    Buffers are device pointers (e.g. torch tensors' data_ptr())."""
+
+    def __init__(self, Mx, My, Mz, precision=PREC_FP64, device=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.spns_fft_create(Mx, My, Mz, precision, device, ctypes.byref(h))
+        if rc:
+            raise SpnsError(rc, self.L.spns_fft_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc:
+            raise SpnsError(rc, self.L.spns_fft_last_error(self.h).decode())
+
+    def r2c(self, rin_ptr, cout_ptr):
+        self._chk(self.L.spns_fft_r2c(self.h, ctypes.c_void_p(rin_ptr), ctypes.c_void_p(cout_ptr)))
+
+    def c2r(self, cin_ptr, rout_ptr):
+        self._chk(self.L.spns_fft_c2r(self.h, ctypes.c_void_p(cin_ptr), ctypes.c_void_p(rout_ptr)))
+
+    def normalize(self, c_ptr, Nx=0, Ny=0, Nz=0):
+        self._chk(self.L.spns_fft_normalize(self.h, ctypes.c_void_p(c_ptr), Nx, Ny, Nz))
+
+    def synchronize(self):
+        self._chk(self.L.spns_fft_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.spns_fft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1075 @@
+// Host side of the MI355X HIT step: context, device memory plan, the step's
+// launch sequence and the extern "C" entry points declared in include/spns.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spns.h"
+#include "spns_internal.hpp"
+
+using namespace spns;
+
+namespace {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                     \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) throw Error(SPNS_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr double kPi = 3.141592653589793238462643;  // src/util.h:57
+
+// exp(-2 pi i m/n) in long double with octant reduction (quarter and eighth
+// points exact by symmetry).
+void twiddle(long m, long n, double* re, double* im)
+{
+    m %= n;
+    if (m < 0) m += n;
+    long a = 8 * m;
+    bool sflip = false, cflip = false, swp = false;
+    if (a > 4 * n) { a = 8 * n - a; sflip = true; }
+    if (a > 2 * n) { a = 4 * n - a; cflip = true; }
+    if (a > n) { a = 2 * n - a; swp = true; }
+    long double th = (long double)kPi * (long double)a / (4.0L * (long double)n);
+    long double c = cosl(th), s = sinl(th);
+    if (a == 0) { c = 1.0L; s = 0.0L; }
+    if (swp) std::swap(c, s);
+    if (cflip) c = -c;
+    if (sflip) s = -s;
+    *re = (double)c;
+    *im = (double)(-s);
+}
+
+void* upload_twiddles(int L, int prec)
+{
+    std::vector<double> t(2 * (size_t)L);
+    for (int m = 0; m < L; m++) twiddle(m, L, &t[2 * m], &t[2 * m + 1]);
+    void* d = nullptr;
+    if (prec == PREC_FP64) {
+        HIPCHK(hipMalloc(&d, sizeof(double) * 2 * L));
+        HIPCHK(hipMemcpy(d, t.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(2 * (size_t)L);
+        for (size_t i = 0; i < f.size(); i++) f[i] = (float)t[i];
+        HIPCHK(hipMalloc(&d, sizeof(float) * 2 * L));
+        HIPCHK(hipMemcpy(d, f.data(), sizeof(float) * 2 * L, hipMemcpyHostToDevice));
+    }
+    return d;
+}
+
+inline int ffreq(int a, int M) { return (a <= M / 2) ? a : a - M; }  // src/util.h:67
+
+// Checks every address a strided launch can touch against the buffers.
+void check_strided(const StridedArgs& a, int A, const int64_t* in_cap, const int64_t* out_cap)
+{
+    for (int j = 0; j < a.njobs; j++) {
+        int64_t imax = (int64_t)(A - 1) * a.in_a_stride + (int64_t)(a.nin - 1) * a.in_row_stride + (a.B - 1);
+        int64_t omax = (int64_t)(A - 1) * a.out_a_stride + (int64_t)(a.nout - 1) * a.out_row_stride + (a.B - 1);
+        if (imax >= in_cap[j] || omax >= out_cap[j] || a.nin < 1 || a.nout < 1 || a.B < 1)
+            throw Error(SPNS_EINVAL, "strided pass would address outside its buffers");
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// FFT_package drop-in
+// ===========================================================================
+struct spns_fft {
+    int Mx, My, Mz, prec, device;
+    void *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    hipStream_t st = nullptr;
+    std::string err;
+
+    void fft_x(void* c, int dir)
+    {
+        const int nk = Mz / 2 + 1;
+        StridedArgs a{};
+        a.job[0] = StridedJob{c, c, 0, 0};
+        a.njobs = 1;
+        a.B = nk; a.nin = Mx; a.nout = Mx;
+        a.in_row_stride = nk; a.out_row_stride = nk;
+        a.in_a_stride = (int64_t)Mx * nk; a.out_a_stride = (int64_t)Mx * nk;
+        a.tw = twx;
+        int64_t cap = (int64_t)My * Mx * nk;
+        check_strided(a, My, &cap, &cap);
+        HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, dir, a, My, st));
+    }
+    void fft_y(void* c, int dir)
+    {
+        const int nk = Mz / 2 + 1;
+        StridedArgs a{};
+        a.job[0] = StridedJob{c, c, 0, 0};
+        a.njobs = 1;
+        a.B = Mx * nk; a.nin = My; a.nout = My;
+        a.in_row_stride = (int64_t)Mx * nk; a.out_row_stride = (int64_t)Mx * nk;
+        a.tw = twy;
+        int64_t cap = (int64_t)My * Mx * nk;
+        check_strided(a, 1, &cap, &cap);
+        HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, dir, a, 1, st));
+    }
+    void z(const void* in, void* out, int dir)
+    {
+        ZPlainArgs a{in, out, Mx, My, twz};
+        int64_t np = (int64_t)Mx * My;
+        int grid = (int)std::min<int64_t>((np + 1) / 2, 65536);
+        HIPCHK(launch_zplain(Mz, prec, dir, a, grid, st));
+    }
+};
+
+// ===========================================================================
+// Solver context
+// ===========================================================================
+struct spns_ctx {
+    spns_config cfg{};
+    int Nx, Ny, Nz, Mx, My, Mz, Nx_2, Ny_2, Nz_2;
+    int nk1, nk2, nk3, nbins, last_rad, last_rad_max, prec, les, rot, sa;
+    int ActualMx, ActualMy, ActualMz;
+    int64_t nmodes;
+    double Ax, Ay, Az, delta, ct, nf;
+    size_t tsz;  // bytes of one complex T
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device memory
+    cx<double>* uk[3] = {};
+    cx<double>* R[3] = {};
+    cx<double>* ukprev[3] = {};
+    void* X1 = nullptr;
+    void* X2 = nullptr;
+    int64_t x1_slot = 0, x2_slot = 0;
+    int n_x1 = 0, nfwd = 3;
+    void *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    DevScalars* scal = nullptr;
+    double* hist_partial = nullptr;
+    double* om_partial = nullptr;
+    double* skew_partial = nullptr;
+    unsigned long long* mv = nullptr;  // [2]
+    double* ek = nullptr;
+    double* forced = nullptr;
+    double* ft = nullptr;
+    int grid_epi = 0, grid_z = 0;
+    hipGraphExec_t graph = nullptr;
+    bool graph_ready = false;
+    std::string err;
+    double last_ms = 0.0;
+
+    char* xbase(void* p, int slot, int64_t slot_elems) const { return (char*)p + (size_t)slot * slot_elems * tsz; }
+    void* X1f(int f) const { return xbase(X1, f, x1_slot); }
+    void* X2f(int f) const { return xbase(X2, f, x2_slot); }
+    void* Cf(int f) const { return xbase(X2, f, nmodes); }  // compact transformed fields, in X2's space
+
+    void init(const spns_config& c)
+    {
+        cfg = c;
+        if (cfg.Lx <= 0) cfg.Lx = 1;
+        if (cfg.Ly <= 0) cfg.Ly = 1;
+        if (cfg.Lz <= 0) cfg.Lz = 1;
+        Nx = cfg.Nx; Ny = cfg.Ny; Nz = cfg.Nz;
+        if (Nx < 2 || Ny < 2 || Nz < 2) throw Error(SPNS_EINVAL, "Nx, Ny, Nz must be >= 2");
+        Mx = cfg.Mx > 0 ? cfg.Mx : (3 * Nx) / 2;
+        My = cfg.My > 0 ? cfg.My : (3 * Ny) / 2;
+        Mz = cfg.Mz > 0 ? cfg.Mz : (3 * Nz) / 2;
+        cfg.Mx = Mx; cfg.My = My; cfg.Mz = Mz;
+        for (int L : {Mx, My, Mz})
+            if (!supported_length(L)) throw Error(SPNS_EUNSUPPORTED, "FFT length " + std::to_string(L) + " is not compiled in");
+        prec = cfg.precision == SPNS_PREC_FP32_SPLIT ? PREC_FP32 : PREC_FP64;
+        tsz = prec == PREC_FP64 ? 16 : 8;
+        Nx_2 = Nx / 2; Ny_2 = Ny / 2; Nz_2 = Nz / 2;
+        nk1 = 2 * Nx_2 + 1; nk2 = 2 * Ny_2 + 1; nk3 = Nz_2 + 1;
+        if (Nx_2 >= (Mx + 1) / 2 || Ny_2 >= (My + 1) / 2 || Nz_2 >= Mz / 2)
+            throw Error(SPNS_EINVAL, "retained modes do not fit the FFT grid");
+        nmodes = (int64_t)nk2 * nk1 * nk3;
+        // src/HIT_LES.h:31-58
+        ActualMx = (Mx % 2 == 0) ? Mx / cfg.Lx : (Mx - 1) / cfg.Lx + 1;
+        ActualMy = (My % 2 == 0) ? My / cfg.Ly : (My - 1) / cfg.Ly + 1;
+        ActualMz = (Mz % 2 == 0) ? Mz / cfg.Lz : (Mz - 1) / cfg.Lz + 1;
+        last_rad = std::min((Nx - 1) / 2, std::min((Ny - 1) / 2, (Nz - 1) / 2));
+        last_rad_max = (int)std::ceil(std::sqrt(1.0 * Nx_2 * Nx_2 + 1.0 * Ny_2 * Ny_2 + 1.0 * Nz_2 * Nz_2));
+        nbins = last_rad_max + 1;
+        delta = cfg.delta > 0 ? cfg.delta
+                              : 2.0 * kPi / std::cbrt((double)(cfg.Lx * cfg.Ly * cfg.Lz) * ActualMx * ActualMy * ActualMz);
+        Ax = 2.0 * kPi / ActualMx / cfg.Lx;
+        Ay = 2.0 * kPi / ActualMy / cfg.Ly;
+        Az = 2.0 * kPi / ActualMz / cfg.Lz;
+        ct = cfg.C_Smag * cfg.C_Smag * delta * delta;  // src/HIT_LES.cpp:132
+        nf = (double)((int64_t)Mx * My * Mz);
+        les = cfg.C_Smag > 0.0;
+        rot = (std::fabs(cfg.omega_x) > 0 || std::fabs(cfg.omega_y) > 0 || std::fabs(cfg.omega_z) > 0);
+        sa = cfg.isSelfAdaptive != 0;
+        nfwd = les ? 9 : 3;
+
+        HIPCHK(hipSetDevice(cfg.device));
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        const size_t sb = sizeof(cx<double>) * (size_t)nmodes;
+        for (int c = 0; c < 3; c++) {
+            alloc((void**)&uk[c], sb);
+            alloc((void**)&R[c], sb);
+            if (sa) alloc((void**)&ukprev[c], sb);
+        }
+        x1_slot = (int64_t)My * nk1 * nk3;
+        x2_slot = (int64_t)My * Mx * nk3;
+        n_x1 = std::max(6, nfwd);
+        alloc(&X1, (size_t)n_x1 * x1_slot * tsz);
+        alloc(&X2, (size_t)9 * x2_slot * tsz);
+        twx = upload_twiddles(Mx, prec);
+        twy = upload_twiddles(My, prec);
+        twz = upload_twiddles(Mz, prec);
+        alloc((void**)&scal, sizeof(DevScalars));
+        grid_epi = (int)std::min<int64_t>((nmodes + 255) / 256, 2048);
+        const int64_t npen = (int64_t)My * Mx;
+        grid_z = (int)std::min<int64_t>(npen, 65536);
+        alloc((void**)&hist_partial, sizeof(double) * (size_t)grid_epi * nbins);
+        alloc((void**)&om_partial, sizeof(double) * grid_epi);
+        alloc((void**)&skew_partial, sizeof(double) * 2 * grid_z);
+        alloc((void**)&mv, sizeof(unsigned long long) * 2);
+        alloc((void**)&ek, sizeof(double) * nbins);
+        alloc((void**)&ft, sizeof(double) * (nbins + 1));
+        HIPCHK(hipMemsetAsync(skew_partial, 0, sizeof(double) * 2 * grid_z, st));
+        HIPCHK(hipMemsetAsync(mv, 0, sizeof(unsigned long long) * 2, st));
+        for (int c = 0; c < 3; c++) {
+            HIPCHK(hipMemsetAsync(uk[c], 0, sb, st));
+            HIPCHK(hipMemsetAsync(R[c], 0, sb, st));
+            if (sa) HIPCHK(hipMemsetAsync(ukprev[c], 0, sb, st));
+        }
+        DevScalars s{};
+        s.nu = cfg.nu;
+        s.kappa = 0.5;  // src/HIT_LES.cpp:1053
+        HIPCHK(hipMemcpyAsync(scal, &s, sizeof(s), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void alloc(void** p, size_t bytes)
+    {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            throw Error(SPNS_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+        }
+    }
+
+    void release()
+    {
+        if (graph) hipGraphExecDestroy(graph);
+        for (int c = 0; c < 3; c++) { hipFree(uk[c]); hipFree(R[c]); hipFree(ukprev[c]); }
+        hipFree(X1); hipFree(X2); hipFree(twx); hipFree(twy); hipFree(twz);
+        hipFree(scal); hipFree(hist_partial); hipFree(om_partial); hipFree(skew_partial); hipFree(mv);
+        hipFree(ek); hipFree(forced); hipFree(ft);
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (st) hipStreamDestroy(st);
+    }
+
+    // ---------------------------------------------------------------- passes
+    void inv_y(bool velocity_only)
+    {
+        StridedArgs a{};
+        int n = 0;
+        for (int c = 0; c < 3; c++) {
+            a.job[n++] = StridedJob{uk[c], X1f(c), 0, 0};
+            if (!velocity_only) a.job[n++] = StridedJob{uk[c], X1f(3 + c), 1, 0};
+        }
+        a.njobs = n;
+        a.B = nk1 * nk3; a.nin = nk2; a.nout = My;
+        a.in_row_stride = (int64_t)nk1 * nk3; a.out_row_stride = (int64_t)nk1 * nk3;
+        a.tw = twy;
+        int64_t icap[9], ocap[9];
+        for (int j = 0; j < n; j++) { icap[j] = nmodes; ocap[j] = x1_slot; }
+        check_strided(a, 1, icap, ocap);
+        HIPCHK(launch_strided(My, prec, 1, prec == PREC_FP64, +1, a, 1, st));
+    }
+
+    void inv_x(bool velocity_only)
+    {
+        StridedArgs a{};
+        int n = 0;
+        for (int c = 0; c < 3; c++) {
+            a.job[n++] = StridedJob{X1f(c), X2f(c), 0, 0};
+            if (!velocity_only) {
+                a.job[n++] = StridedJob{X1f(c), X2f(3 + c), 1, 0};
+                a.job[n++] = StridedJob{X1f(3 + c), X2f(6 + c), 0, 0};
+            }
+        }
+        a.njobs = n;
+        a.B = nk3; a.nin = nk1; a.nout = Mx;
+        a.in_row_stride = nk3; a.in_a_stride = (int64_t)nk1 * nk3;
+        a.out_row_stride = nk3; a.out_a_stride = (int64_t)Mx * nk3;
+        a.tw = twx;
+        int64_t icap[9], ocap[9];
+        for (int j = 0; j < n; j++) { icap[j] = x1_slot; ocap[j] = x2_slot; }
+        check_strided(a, My, icap, ocap);
+        HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, +1, a, My, st));
+    }
+
+    void zpass(int mode, void* phys)
+    {
+        ZArgs z{};
+        for (int f = 0; f < 9; f++) z.f[f] = X2f(f);
+        z.nk3 = nk3;
+        z.les = les;
+        z.npencil = (int64_t)My * Mx;
+        z.Ax = Ax; z.Ay = Ay; z.Az = Az;
+        z.ct = ct;
+        z.maxvel_bits = mv;
+        z.maxvel_sa_bits = mv + 1;
+        z.skew_partial = skew_partial;
+        z.phys = phys;
+        z.tw = twz;
+        if (z.npencil * nk3 > x2_slot) throw Error(SPNS_EINVAL, "z pass exceeds its buffer");
+        HIPCHK(launch_zpass(Mz, prec, mode, z, grid_z, st));
+    }
+
+    void fwd_x()
+    {
+        StridedArgs a{};
+        for (int f = 0; f < nfwd; f++) a.job[f] = StridedJob{X2f(f), X1f(f), 0, 0};
+        a.njobs = nfwd;
+        a.B = nk3; a.nin = Mx; a.nout = nk1;
+        a.in_row_stride = nk3; a.in_a_stride = (int64_t)Mx * nk3;
+        a.out_row_stride = nk3; a.out_a_stride = (int64_t)nk1 * nk3;
+        a.tw = twx;
+        int64_t icap[9], ocap[9];
+        for (int j = 0; j < nfwd; j++) { icap[j] = x2_slot; ocap[j] = x1_slot; }
+        check_strided(a, My, icap, ocap);
+        HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, My, st));
+    }
+
+    void fwd_y()
+    {
+        StridedArgs a{};
+        for (int f = 0; f < nfwd; f++) a.job[f] = StridedJob{X1f(f), Cf(f), 0, 0};
+        a.njobs = nfwd;
+        a.B = nk1 * nk3; a.nin = My; a.nout = nk2;
+        a.in_row_stride = (int64_t)nk1 * nk3; a.out_row_stride = (int64_t)nk1 * nk3;
+        a.tw = twy;
+        int64_t icap[9], ocap[9];
+        for (int j = 0; j < nfwd; j++) { icap[j] = x1_slot; ocap[j] = nmodes; }
+        check_strided(a, 1, icap, ocap);
+        if ((int64_t)nfwd * nmodes > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
+        HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, 1, st));
+    }
+
+    EpiArgs epi_args(int mode)
+    {
+        EpiArgs e{};
+        for (int f = 0; f < nfwd; f++) e.conv[f] = Cf(f);
+        for (int c = 0; c < 3; c++) { e.uk[c] = uk[c]; e.R[c] = R[c]; e.ukprev[c] = ukprev[c]; }
+        e.nk1 = nk1; e.nk2 = nk2; e.nk3 = nk3; e.nbins = nbins;
+        e.nmodes = nmodes;
+        e.notcubic = cfg.isNotCubic; e.Lx = cfg.Lx; e.Ly = cfg.Ly; e.Lz = cfg.Lz; e.Mz = Mz;
+        e.mode = mode; e.les = les; e.rot = rot; e.force_on = cfg.force_on;
+        e.kfmin = cfg.force_kmin; e.kfmax = cfg.force_kmax; e.sa = sa;
+        e.nf = nf;
+        e.ox = cfg.omega_x; e.oy = cfg.omega_y; e.oz = cfg.omega_z;
+        e.scal = scal;
+        e.hist_partial = hist_partial;
+        e.om_partial = om_partial;
+        return e;
+    }
+
+    FinArgs fin_args(int flags)
+    {
+        FinArgs f{};
+        f.scal = scal;
+        f.flags = flags;
+        f.nbins = nbins; f.nblk_epi = grid_epi; f.nblk_z = grid_z;
+        f.hist_partial = hist_partial; f.om_partial = om_partial; f.skew_partial = skew_partial;
+        f.maxvel_bits = mv; f.maxvel_sa_bits = mv + 1;
+        f.ek = ek;
+        f.kfmin = cfg.force_kmin; f.kfmax = cfg.force_kmax; f.force_eps = cfg.force_on ? cfg.force_eps : 0.0;
+        f.Ax = Ax; f.Ay = Ay; f.Az = Az; f.C_At = cfg.C_At; f.fixed_dt = cfg.fixed_dt;
+        f.sa = sa;
+        // self-adaptive constants (src/HIT_LES.cpp:1049-1078)
+        f.Ah = (Ax + Ay + Az) / 3.0;
+        f.t1 = 0.9302468;
+        f.kk[0] = 0.73782212; f.kk[1] = 0.44660387;
+        f.phi[0] = std::atan(164.0 / 99.0); f.phi[1] = kPi / 3.0; f.phi[2] = kPi * 3.0 * 3.0 / (5.0 * 5.0);
+        const double c[14] = {0.0647998, -0.386022, 3.72945, -9.38143, 7.06574, 2403400.0, -5018490.0,
+                              2620140.0, 2945.0, -6665.76, 3790.54, 4.80513, -16.9473, 15.0155};
+        std::memcpy(f.c, c, sizeof(c));
+        return f;
+    }
+
+    void enqueue_step()
+    {
+        HIPCHK(launch_clear(mv, mv + 1, st));
+        inv_y(false);
+        inv_x(false);
+        zpass(0, nullptr);
+        fwd_x();
+        fwd_y();
+        HIPCHK(launch_epilogue(prec, epi_args(EPI_STEP), grid_epi, st));
+        HIPCHK(launch_finalize(fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE), st));
+    }
+
+    void reduce_state()
+    {
+        HIPCHK(launch_epilogue(prec, epi_args(EPI_REDUCE), grid_epi, st));
+        HIPCHK(launch_finalize(fin_args(FIN_REDUCE), st));
+    }
+
+    // u,v,w from the state: CFL maxima (and the physical field if phys)
+    void velocity_pass(void* phys)
+    {
+        HIPCHK(launch_clear(mv, mv + 1, st));
+        inv_y(true);
+        inv_x(true);
+        zpass(1, phys);
+        HIPCHK(launch_finalize(fin_args(FIN_MAXVEL), st));
+    }
+
+    void step(int n)
+    {
+        HIPCHK(hipEventRecord(ev0, st));
+        if (cfg.use_graph) {
+            if (!graph_ready) {
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_step();
+                } catch (...) {
+                    hipGraph_t dummy;
+                    hipStreamEndCapture(st, &dummy);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(st, &g));
+                HIPCHK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+                HIPCHK(hipGraphDestroy(g));
+                graph_ready = true;
+            }
+            for (int i = 0; i < n; i++) HIPCHK(hipGraphLaunch(graph, st));
+        } else {
+            for (int i = 0; i < n; i++) enqueue_step();
+        }
+        HIPCHK(hipEventRecord(ev1, st));
+    }
+
+    DevScalars read_scalars()
+    {
+        DevScalars s;
+        HIPCHK(hipMemcpyAsync(&s, scal, sizeof(s), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return s;
+    }
+
+    void write_scalars(const DevScalars& s)
+    {
+        HIPCHK(hipMemcpyAsync(scal, &s, sizeof(s), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void upload_state(const std::vector<std::complex<double>>& h)
+    {
+        // h: [3][nmodes]; modes masked by the length factors are zeroed
+        std::vector<std::complex<double>> m(h);
+        if (cfg.isNotCubic) {
+            for (int64_t q = 0; q < nmodes; q++) {
+                int i3 = (int)(q % nk3);
+                int64_t r = q / nk3;
+                int i1 = (int)(r % nk1), i2 = (int)(r / nk1);
+                int k1 = i1 <= nk1 / 2 ? i1 : i1 - nk1, k2 = i2 <= nk2 / 2 ? i2 : i2 - nk2, k3 = i3;
+                if ((k1 % cfg.Lx) || (k2 % cfg.Ly) || (k3 % cfg.Lz))
+                    for (int c = 0; c < 3; c++) m[c * nmodes + q] = 0.0;
+            }
+        }
+        for (int c = 0; c < 3; c++)
+            HIPCHK(hipMemcpyAsync(uk[c], m.data() + c * nmodes, sizeof(cx<double>) * nmodes, hipMemcpyHostToDevice, st));
+        reset_history();
+    }
+
+    void reset_history()
+    {
+        for (int c = 0; c < 3; c++) {
+            HIPCHK(hipMemsetAsync(R[c], 0, sizeof(cx<double>) * nmodes, st));
+            if (sa) HIPCHK(hipMemsetAsync(ukprev[c], 0, sizeof(cx<double>) * nmodes, st));
+        }
+        DevScalars s = read_scalars();
+        s.time = 0.0;
+        s.iter = 0;
+        write_scalars(s);
+    }
+
+    std::vector<std::complex<double>> download_state(cx<double>* const* src)
+    {
+        std::vector<std::complex<double>> h(3 * (size_t)nmodes);
+        for (int c = 0; c < 3; c++)
+            HIPCHK(hipMemcpyAsync(h.data() + c * nmodes, src[c], sizeof(cx<double>) * nmodes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return h;
+    }
+
+    int64_t compact_index(int k1, int k2, int k3) const
+    {
+        int i1 = k1 >= 0 ? k1 : nk1 + k1, i2 = k2 >= 0 ? k2 : nk2 + k2;
+        return ((int64_t)i2 * nk1 + i1) * nk3 + k3;
+    }
+    bool retained(int k1, int k2, int k3) const
+    {
+        if (std::abs(k1) > Nx_2 || std::abs(k2) > Ny_2 || k3 > Nz_2 || k3 < 0) return false;
+        if (cfg.isNotCubic && ((k1 % cfg.Lx) || (k2 % cfg.Ly) || (k3 % cfg.Lz))) return false;
+        return true;
+    }
+
+    void forcing_energy_cascade(const double* fek, int64_t last_input_rad, int nullify)
+    {
+        reduce_state();
+        const int last_eff = (int)std::min<int64_t>(last_input_rad, last_rad);
+        if (last_eff < 0) return;
+        std::vector<double> f(fek, fek + last_eff + 1);
+        if (forced) hipFree(forced);
+        forced = nullptr;
+        alloc((void**)&forced, sizeof(double) * (last_eff + 1));
+        HIPCHK(hipMemcpyAsync(forced, f.data(), sizeof(double) * (last_eff + 1), hipMemcpyHostToDevice, st));
+        ShellArgs s = shell_args();
+        s.ft = ft;
+        s.last_eff = last_eff;
+        s.nullify = nullify;
+        if (last_eff + 1 > nbins + 1) throw Error(SPNS_EINVAL, "forcing shells exceed the spectrum");
+        HIPCHK(launch_forcing(ek, forced, ft, last_eff, s, st));
+        if (cfg.isComplexConjugateCorrected) HIPCHK(launch_ccc(shell_args(), st));
+        reduce_state();
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    ShellArgs shell_args()
+    {
+        ShellArgs s{};
+        for (int c = 0; c < 3; c++) s.uk[c] = uk[c];
+        s.nk1 = nk1; s.nk2 = nk2; s.nk3 = nk3; s.nmodes = nmodes;
+        s.notcubic = cfg.isNotCubic; s.Lx = cfg.Lx; s.Ly = cfg.Ly; s.Lz = cfg.Lz;
+        return s;
+    }
+};
+
+namespace {
+
+// Counter-based generator for the synthetic initial field: splitmix64 of a
+// key built from (seed, k1, k2, k3, draw), so the field is independent of
+// any decomposition or batch.
+inline uint64_t splitmix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+inline double unif(uint64_t seed, int k1, int k2, int k3, int d)
+{
+    uint64_t key = splitmix(seed) ^ splitmix(((uint64_t)(uint32_t)(k1 + 65536) << 40) ^ ((uint64_t)(uint32_t)(k2 + 65536) << 20) ^
+                                             (uint64_t)(uint32_t)k3 ^ ((uint64_t)d << 58));
+    return ((double)(splitmix(key) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+int finish(spns_ctx* c, const std::exception& e, int code)
+{
+    if (c) c->err = e.what();
+    return code;
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+#define API_BEGIN(c) \
+    if (!(c)) return SPNS_EINVAL; \
+    try {
+#define API_END(c)                                   \
+    }                                                \
+    catch (const Error& e) { return finish(c, e, e.code); } \
+    catch (const std::exception& e) { return finish(c, e, SPNS_EINVAL); } \
+    return SPNS_OK;
+
+extern "C" {
+
+int spns_create(const spns_config* cfg, spns_ctx** out)
+{
+    if (!cfg || !out) return SPNS_EINVAL;
+    *out = nullptr;
+    spns_ctx* c = new spns_ctx();
+    try {
+        c->init(*cfg);
+    } catch (const Error& e) {
+        g_create_error = e.what();
+        c->release();
+        delete c;
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        c->release();
+        delete c;
+        return SPNS_EINVAL;
+    }
+    *out = c;
+    return SPNS_OK;
+}
+
+int spns_destroy(spns_ctx* ctx)
+{
+    if (!ctx) return SPNS_EINVAL;
+    hipSetDevice(ctx->cfg.device);
+    hipStreamSynchronize(ctx->st);
+    ctx->release();
+    delete ctx;
+    return SPNS_OK;
+}
+
+int64_t spns_nmodes(const spns_ctx* ctx) { return ctx ? ctx->nmodes : -1; }
+int32_t spns_nbins(const spns_ctx* ctx) { return ctx ? ctx->nbins : -1; }
+int32_t spns_last_rad(const spns_ctx* ctx) { return ctx ? ctx->last_rad : -1; }
+const char* spns_last_error(const spns_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int spns_set_state(spns_ctx* ctx, const double* state)
+{
+    API_BEGIN(ctx)
+    if (!state) throw Error(SPNS_EINVAL, "null state");
+    std::vector<std::complex<double>> h(3 * (size_t)ctx->nmodes);
+    std::memcpy(h.data(), state, sizeof(double) * 6 * ctx->nmodes);
+    ctx->upload_state(h);
+    API_END(ctx)
+}
+
+int spns_get_state(spns_ctx* ctx, double* state)
+{
+    API_BEGIN(ctx)
+    if (!state) throw Error(SPNS_EINVAL, "null output");
+    auto h = ctx->download_state(ctx->uk);
+    std::memcpy(state, h.data(), sizeof(double) * 6 * ctx->nmodes);
+    API_END(ctx)
+}
+
+int spns_get_rhist(spns_ctx* ctx, double* rh)
+{
+    API_BEGIN(ctx)
+    if (!rh) throw Error(SPNS_EINVAL, "null output");
+    auto h = ctx->download_state(ctx->R);
+    std::memcpy(rh, h.data(), sizeof(double) * 6 * ctx->nmodes);
+    API_END(ctx)
+}
+
+int spns_init_from_physical(spns_ctx* ctx, const double* u, const double* v, const double* w, int32_t nxf, int32_t nyf,
+                            int32_t nzf, double* ek_file, double* ens_file)
+{
+    API_BEGIN(ctx)
+    if (!u || !v || !w || nxf < 2 || nyf < 2 || nzf < 2) throw Error(SPNS_EINVAL, "bad file field");
+    // r2c of the file grid on the device (src/HIT_LES.cpp:557-583)
+    spns_fft* f = nullptr;
+    int rc = spns_fft_create(nxf, nyf, nzf, SPNS_PREC_FP64, ctx->cfg.device, &f);
+    if (rc) throw Error(rc, "file grid: FFT length not compiled in");
+    const int nzh = nzf / 2 + 1;
+    const size_t nr = (size_t)nxf * nyf * 2 * nzh, ncf = (size_t)nyf * nxf * nzh;
+    std::vector<double> pad(nr, 0.0);
+    std::vector<std::complex<double>> spec[3];
+    double *dr = nullptr, *dc = nullptr;
+    try {
+        HIPCHK(hipMalloc((void**)&dr, sizeof(double) * nr));
+        HIPCHK(hipMalloc((void**)&dc, sizeof(double) * 2 * ncf));
+        const double* src[3] = {u, v, w};
+        for (int c = 0; c < 3; c++) {
+            for (int i = 0; i < nxf; i++)
+                for (int j = 0; j < nyf; j++)
+                    for (int k = 0; k < nzf; k++)
+                        pad[((size_t)i * nyf + j) * 2 * nzh + k] = src[c][((size_t)i * nyf + j) * nzf + k];
+            HIPCHK(hipMemcpy(dr, pad.data(), sizeof(double) * nr, hipMemcpyHostToDevice));
+            if ((rc = spns_fft_r2c(f, dr, dc))) throw Error(rc, spns_fft_last_error(f));
+            spec[c].resize(ncf);
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipMemcpy(spec[c].data(), dc, sizeof(double) * 2 * ncf, hipMemcpyDeviceToHost));
+            const double norm = (double)(nxf * nyf * nzf);
+            for (auto& z : spec[c]) z = std::complex<double>(z.real() / norm, z.imag() / norm);
+        }
+    } catch (...) {
+        hipFree(dr); hipFree(dc);
+        spns_fft_destroy(f);
+        throw;
+    }
+    hipFree(dr); hipFree(dc);
+    spns_fft_destroy(f);
+    // file spectrum energy (src/HIT_LES.cpp:1268-1294); spectrum layout [y][x][z]
+    double ekf = 0.0, ensf = 0.0;
+    for (int a = 0; a < nxf; a++)
+        for (int b = 0; b < nyf; b++)
+            for (int k3 = 0; k3 < nzh; k3++) {
+                size_t id = ((size_t)b * nxf + a) * nzh + k3;
+                int k1 = ffreq(a, nxf), k2 = ffreq(b, nyf);
+                int r2 = k1 * k1 + k2 * k2 + k3 * k3;
+                if (!((k1 % ctx->cfg.Lx == 0) && (k2 % ctx->cfg.Ly == 0) && (k3 % ctx->cfg.Lz == 0))) continue;
+                bool conj = (k3 != 0) && !((nzf % 2 == 0) && (k3 == nzf / 2));
+                double e = (spec[0][id].real() * spec[0][id].real()) + (spec[0][id].imag() * spec[0][id].imag());
+                e += (spec[1][id].real() * spec[1][id].real()) + (spec[1][id].imag() * spec[1][id].imag());
+                e += (spec[2][id].real() * spec[2][id].real()) + (spec[2][id].imag() * spec[2][id].imag());
+                e *= 0.5;
+                if (conj) { ekf += 2 * e; ensf += 2 * (2 * r2 * e); }
+                else { ekf += e; ensf += (2 * r2 * e); }
+            }
+    if (ek_file) *ek_file = ekf;
+    if (ens_file) *ens_file = ensf;
+    // truncation (src/HIT_LES.cpp:738-763)
+    std::vector<std::complex<double>> h(3 * (size_t)ctx->nmodes, 0.0);
+    for (int k2 = -ctx->Ny_2; k2 <= ctx->Ny_2; k2++)
+        for (int k1 = -ctx->Nx_2; k1 <= ctx->Nx_2; k1++)
+            for (int k3 = 0; k3 <= ctx->Nz_2; k3++) {
+                if (!ctx->retained(k1, k2, k3)) continue;
+                if (!((std::abs(k1) <= (nxf - 1) / 2) && (std::abs(k2) <= (nyf - 1) / 2) && (k3 <= (nzf - 1) / 2))) continue;
+                int af = k1 >= 0 ? k1 : nxf + k1, bf = k2 >= 0 ? k2 : nyf + k2;
+                size_t idf = ((size_t)bf * nxf + af) * nzh + k3;
+                int64_t q = ctx->compact_index(k1, k2, k3);
+                for (int c = 0; c < 3; c++) h[c * ctx->nmodes + q] = spec[c][idf];
+            }
+    ctx->upload_state(h);
+    API_END(ctx)
+}
+
+int spns_init_k41(spns_ctx* ctx, int32_t nranks)
+{
+    API_BEGIN(ctx)
+    if (nranks < 1) throw Error(SPNS_EINVAL, "nranks must be >= 1");
+    const int Mx = ctx->Mx, My = ctx->My, Mz2 = ctx->Mz / 2;
+    std::vector<std::complex<double>> h(3 * (size_t)ctx->nmodes, 0.0);
+    const int block = (My + nranks - 1) / nranks;
+    // every rank draws over its own y-planes (src/HIT_LES.cpp:504-525)
+    for (int r = 0; r < nranks; r++) {
+        const int b0 = r * block, b1 = std::min(My, b0 + block);
+        srand(0);
+        for (int b = b0; b < b1; b++)
+            for (int a = 0; a < Mx; a++)
+                for (int k3 = 0; k3 <= Mz2; k3++) {
+                    const int k1 = ffreq(a, Mx), k2 = ffreq(b, My);
+                    const bool d = ctx->retained(k1, k2, k3);
+                    double val[2][3] = {{0, 0, 0}, {0, 0, 0}};
+                    for (int ic = 0; ic <= 1; ic++) {
+                        if (d) {
+                            int r2 = k1 * k1 + k2 * k2 + k3 * k3;
+                            int K = (int)std::lround(std::sqrt((double)r2));
+                            if (K <= ctx->last_rad)
+                                for (int c = 0; c < 3; c++) val[ic][c] = (((double)rand() / RAND_MAX) * 2.0 - 1.0);
+                        }
+                    }
+                    if (d) {
+                        int64_t q = ctx->compact_index(k1, k2, k3);
+                        for (int c = 0; c < 3; c++) h[c * ctx->nmodes + q] = std::complex<double>(val[0][c], val[1][c]);
+                    }
+                }
+    }
+    ctx->upload_state(h);
+    HIPCHK(launch_ccc(ctx->shell_args(), ctx->st));  // src/HIT_LES.cpp:527
+    std::vector<double> k41(ctx->last_rad + 1, 0.0);
+    for (int K = 1; K <= ctx->last_rad; K++) k41[K] = std::pow(K, -5.0 / 3.0);
+    ctx->forcing_energy_cascade(k41.data(), ctx->last_rad, 0);
+    API_END(ctx)
+}
+
+int spns_init_random(spns_ctx* ctx, uint64_t seed, double k0, double urms)
+{
+    API_BEGIN(ctx)
+    if (!(k0 > 0) || !(urms >= 0)) throw Error(SPNS_EINVAL, "k0 must be > 0 and urms >= 0");
+    const int64_t nm = ctx->nmodes;
+    std::vector<std::complex<double>> h(3 * (size_t)nm, 0.0);
+    double etot = 0.0;
+    for (int k2 = -ctx->Ny_2; k2 <= ctx->Ny_2; k2++)
+        for (int k1 = -ctx->Nx_2; k1 <= ctx->Nx_2; k1++)
+            for (int k3 = 0; k3 <= ctx->Nz_2; k3++) {
+                if (!ctx->retained(k1, k2, k3)) continue;
+                const bool canon = k3 > 0 || k1 > 0 || (k1 == 0 && k2 > 0);
+                if (!canon) continue;
+                const double kk[3] = {(double)k1, (double)k2, (double)k3};
+                const double r2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+                const double kmag = std::sqrt(r2);
+                std::complex<double> g[3];
+                for (int c = 0; c < 3; c++) {
+                    double u1 = unif(seed, k1, k2, k3, 2 * c), u2 = unif(seed, k1, k2, k3, 2 * c + 1);
+                    double rr = std::sqrt(-2.0 * std::log(u1));
+                    g[c] = std::complex<double>(rr * std::cos(2 * kPi * u2), rr * std::sin(2 * kPi * u2));
+                }
+                std::complex<double> kd = (kk[0] * g[0] + kk[1] * g[1] + kk[2] * g[2]) / r2;
+                double nrm = 0.0;
+                for (int c = 0; c < 3; c++) { g[c] -= kd * kk[c]; nrm += std::norm(g[c]); }
+                if (nrm <= 0) continue;
+                const double Ek = std::pow(kmag, 4.0) * std::exp(-2.0 * (kmag / k0) * (kmag / k0));
+                const double amp = std::sqrt(Ek / (4.0 * kPi * r2) / nrm);
+                const int64_t q = ctx->compact_index(k1, k2, k3);
+                for (int c = 0; c < 3; c++) h[c * nm + q] = g[c] * amp;
+                if (k3 == 0) {
+                    const int64_t qm = ctx->compact_index(-k1, -k2, 0);
+                    for (int c = 0; c < 3; c++) h[c * nm + qm] = std::conj(g[c] * amp);
+                }
+            }
+    for (int64_t q = 0; q < nm; q++) {
+        int i3 = (int)(q % ctx->nk3);
+        double w = (i3 != 0) ? 2.0 : 1.0;
+        double ke = 0.5 * (std::norm(h[q]) + std::norm(h[nm + q]) + std::norm(h[2 * nm + q]));
+        etot += w * ke;
+    }
+    const double target = 1.5 * urms * urms;
+    const double sc = etot > 0 ? std::sqrt(target / etot) : 0.0;
+    for (auto& z : h) z *= sc;
+    ctx->upload_state(h);
+    API_END(ctx)
+}
+
+int spns_finish_init(spns_ctx* ctx, int32_t isReLambda)
+{
+    API_BEGIN(ctx)
+    ctx->reset_history();
+    DevScalars s = ctx->read_scalars();
+    s.nu = ctx->cfg.nu;
+    s.kappa = 0.5;
+    ctx->write_scalars(s);
+    ctx->velocity_pass(nullptr);
+    if (!isReLambda) HIPCHK(launch_finalize(ctx->fin_args(FIN_DT), ctx->st));
+    ctx->reduce_state();
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    API_END(ctx)
+}
+
+int spns_step(spns_ctx* ctx, int32_t nsteps)
+{
+    API_BEGIN(ctx)
+    if (nsteps < 0) throw Error(SPNS_EINVAL, "nsteps < 0");
+    ctx->step(nsteps);
+    API_END(ctx)
+}
+
+int spns_recalculate_timestep(spns_ctx* ctx)
+{
+    API_BEGIN(ctx)
+    HIPCHK(launch_finalize(ctx->fin_args(FIN_DT), ctx->st));
+    API_END(ctx)
+}
+
+int spns_set_nu(spns_ctx* ctx, double nu)
+{
+    API_BEGIN(ctx)
+    DevScalars s = ctx->read_scalars();
+    s.nu = nu;
+    ctx->cfg.nu = nu;
+    ctx->write_scalars(s);
+    API_END(ctx)
+}
+
+int spns_forcing_reynolds_lambda(spns_ctx* ctx, double relambda)
+{
+    API_BEGIN(ctx)
+    DevScalars s = ctx->read_scalars();
+    s.nu = s.ek / relambda * std::sqrt(20.0 / 3.0 / s.om);  // src/HIT_LES.cpp:488
+    ctx->cfg.nu = s.nu;
+    ctx->write_scalars(s);
+    API_END(ctx)
+}
+
+int spns_recalculate_reynolds_lambda(spns_ctx* ctx, double* relambda)
+{
+    API_BEGIN(ctx)
+    DevScalars s = ctx->read_scalars();
+    double r = s.ek / s.nu * std::sqrt(20.0 / 3.0 / s.om);
+    if (relambda) *relambda = r;
+    API_END(ctx)
+}
+
+int spns_forcing_energy_cascade(spns_ctx* ctx, const double* fek, int64_t last_input_rad, int32_t nullify)
+{
+    API_BEGIN(ctx)
+    if (!fek || last_input_rad < 0) throw Error(SPNS_EINVAL, "bad forced spectrum");
+    ctx->forcing_energy_cascade(fek, last_input_rad, nullify);
+    API_END(ctx)
+}
+
+int spns_complex_conjugate_correction(spns_ctx* ctx)
+{
+    API_BEGIN(ctx)
+    HIPCHK(launch_ccc(ctx->shell_args(), ctx->st));
+    ctx->reduce_state();
+    API_END(ctx)
+}
+
+int spns_energy_cascade(spns_ctx* ctx, double* ekh, int32_t nbins, double* ek_tot)
+{
+    API_BEGIN(ctx)
+    if (ekh && nbins > ctx->nbins) throw Error(SPNS_EINVAL, "nbins exceeds the spectrum");
+    if (ekh && nbins > 0) {
+        HIPCHK(hipMemcpyAsync(ekh, ctx->ek, sizeof(double) * nbins, hipMemcpyDeviceToHost, ctx->st));
+    }
+    DevScalars s = ctx->read_scalars();
+    if (ek_tot) *ek_tot = s.ek;
+    API_END(ctx)
+}
+
+int spns_get_scalars(spns_ctx* ctx, spns_scalars* out)
+{
+    API_BEGIN(ctx)
+    if (!out) throw Error(SPNS_EINVAL, "null output");
+    DevScalars s = ctx->read_scalars();
+    out->time = s.time;
+    out->dt = s.dt;
+    out->nu = s.nu;
+    out->maxvel = ctx->sa ? s.maxvel_sa : s.maxvel;
+    out->ek = s.ek;
+    out->om = s.om;
+    out->relambda = s.ek / s.nu * std::sqrt(20.0 / 3.0 / s.om);
+    const double P = ctx->nf;
+    const double m2 = s.skew2 / P, m3 = s.skew3 / P;
+    out->skewness = m2 > 0 ? m3 / std::pow(m2, 1.5) : 0.0;
+    out->kappa = s.kappa;
+    out->iter = s.iter;
+    if (!std::isfinite(s.maxvel) || !std::isfinite(s.ek)) throw Error(SPNS_ENONFINITE, "non-finite velocity field");
+    API_END(ctx)
+}
+
+int spns_velocity_physical(spns_ctx* ctx, double* uvw)
+{
+    API_BEGIN(ctx)
+    if (!uvw) throw Error(SPNS_EINVAL, "null output");
+    const int Mx = ctx->Mx, My = ctx->My, Mz = ctx->Mz;
+    const size_t P = (size_t)Mx * My * Mz;
+    const size_t ts = ctx->prec == PREC_FP64 ? 8 : 4;
+    void* d = nullptr;
+    ctx->alloc(&d, 3 * P * ts);
+    try {
+        ctx->velocity_pass(d);
+        std::vector<char> hbuf(3 * P * ts);
+        HIPCHK(hipMemcpyAsync(hbuf.data(), d, 3 * P * ts, hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        for (int c = 0; c < 3; c++)
+            for (int y = 0; y < My; y++)
+                for (int x = 0; x < Mx; x++)
+                    for (int z = 0; z < Mz; z++) {
+                        size_t s = c * P + ((size_t)y * Mx + x) * Mz + z;
+                        double v = ts == 8 ? ((double*)hbuf.data())[s] : (double)((float*)hbuf.data())[s];
+                        uvw[c * P + ((size_t)x * My + y) * Mz + z] = v;
+                    }
+    } catch (...) {
+        hipFree(d);
+        throw;
+    }
+    hipFree(d);
+    API_END(ctx)
+}
+
+int spns_synchronize(spns_ctx* ctx)
+{
+    API_BEGIN(ctx)
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    API_END(ctx)
+}
+
+int spns_last_step_ms(spns_ctx* ctx, double* ms)
+{
+    API_BEGIN(ctx)
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+    if (ms) *ms = f;
+    API_END(ctx)
+}
+
+// ------------------------------ FFT_package ------------------------------
+int spns_fft_create(int32_t Mx, int32_t My, int32_t Mz, int32_t precision, int32_t device, spns_fft** out)
+{
+    if (!out) return SPNS_EINVAL;
+    *out = nullptr;
+    if (!supported_length(Mx) || !supported_length(My) || !supported_length(Mz)) {
+        g_create_error = "FFT length not compiled in";
+        return SPNS_EUNSUPPORTED;
+    }
+    spns_fft* f = new spns_fft();
+    try {
+        f->Mx = Mx; f->My = My; f->Mz = Mz; f->device = device;
+        f->prec = precision == SPNS_PREC_FP32_SPLIT ? PREC_FP32 : PREC_FP64;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&f->st, hipStreamNonBlocking));
+        f->twx = upload_twiddles(Mx, f->prec);
+        f->twy = upload_twiddles(My, f->prec);
+        f->twz = upload_twiddles(Mz, f->prec);
+    } catch (const Error& e) {
+        g_create_error = e.what();
+        spns_fft_destroy(f);
+        return e.code;
+    }
+    *out = f;
+    return SPNS_OK;
+}
+
+int spns_fft_destroy(spns_fft* f)
+{
+    if (!f) return SPNS_EINVAL;
+    if (f->st) hipStreamSynchronize(f->st);
+    hipFree(f->twx); hipFree(f->twy); hipFree(f->twz);
+    if (f->st) hipStreamDestroy(f->st);
+    delete f;
+    return SPNS_OK;
+}
+
+int spns_fft_r2c(spns_fft* f, void* rin, void* cout)
+{
+    if (!f || !rin || !cout) return SPNS_EINVAL;
+    try {
+        f->z(rin, cout, -1);
+        f->fft_y(cout, -1);
+        f->fft_x(cout, -1);
+    } catch (const Error& e) {
+        f->err = e.what();
+        return e.code;
+    }
+    return SPNS_OK;
+}
+
+int spns_fft_c2r(spns_fft* f, void* cin, void* rout)
+{
+    if (!f || !cin || !rout) return SPNS_EINVAL;
+    try {
+        f->fft_x(cin, +1);
+        f->fft_y(cin, +1);
+        f->z(cin, rout, +1);
+    } catch (const Error& e) {
+        f->err = e.what();
+        return e.code;
+    }
+    return SPNS_OK;
+}
+
+int spns_fft_normalize(spns_fft* f, void* c, int32_t Nx, int32_t Ny, int32_t Nz)
+{
+    if (!f || !c) return SPNS_EINVAL;
+    try {
+        NormArgs a{};
+        a.data = c;
+        a.Mx = f->Mx; a.My = f->My; a.Mz = f->Mz;
+        a.Nx2 = Nx / 2; a.Ny2 = Ny / 2; a.Nz2 = Nz / 2;
+        a.all = Nx <= 0;
+        a.Lx = a.Ly = a.Lz = 1;
+        a.nf = (double)((int64_t)f->Mx * f->My * f->Mz);
+        HIPCHK(launch_normalize(f->prec, a, f->st));
+    } catch (const Error& e) {
+        f->err = e.what();
+        return e.code;
+    }
+    return SPNS_OK;
+}
+
+int spns_fft_synchronize(spns_fft* f)
+{
+    if (!f) return SPNS_EINVAL;
+    hipError_t e = hipStreamSynchronize(f->st);
+    if (e != hipSuccess) {
+        f->err = hipGetErrorString(e);
+        return SPNS_EHIP;
+    }
+    return SPNS_OK;
+}
+
+const char* spns_fft_last_error(const spns_fft* f) { return f ? f->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

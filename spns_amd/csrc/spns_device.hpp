@@ -1,0 +1,281 @@
+// Device building blocks for the MI355X (gfx950) pseudospectral step:
+// complex arithmetic, small in-register DFTs and a batched Stockham FFT that
+// runs its stages through LDS.
+//
+// The FFT is the Stockham autosort formulation (Govindaraju et al., "High
+// performance discrete Fourier transforms on graphics processors", SC08):
+// stage s with radix R and span Ns reads x[j + r*L/R], multiplies by
+// W_{Ns*R}^{r*(j mod Ns)}, does an R-point DFT and writes
+// y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].  A block transforms NB independent
+// columns of length L that live in LDS as [row][ld] (ld >= NB).  The first
+// stage reads its inputs through a caller functor (global memory, pruned
+// zero padding) and the last stage hands its outputs to another functor, so
+// a transform costs (stages-1) LDS round trips.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spns_internal.hpp"
+
+namespace spns {
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) { return cx<T>{a, b}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return cx<T>{a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return cx<T>{a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) { return cx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return cx<T>{a.x * s, a.y * s}; }
+// multiply by -i (FWD) or +i (INV): dir = -1 forward, +1 inverse
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> mul_i(cx<T> a) { return DIR < 0 ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x}; }
+
+// ---------------------------------------------------------------------------
+// Small DFTs in registers, y_k = sum_n x_n exp(DIR*2*pi*i*n*k/R).
+// ---------------------------------------------------------------------------
+template <int DIR, typename T>
+__device__ __forceinline__ void dft2(cx<T>& a, cx<T>& b)
+{
+    cx<T> t = a - b;
+    a = a + b;
+    b = t;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft3(cx<T>& x0, cx<T>& x1, cx<T>& x2)
+{
+    const T s = (T)0.86602540378443864676;  // sqrt(3)/2
+    cx<T> t1 = x1 + x2;
+    cx<T> t2 = x0 - cscale(t1, (T)0.5);
+    cx<T> t3 = cscale(mul_i<DIR>(x1 - x2), s);
+    x0 = x0 + t1;
+    x1 = t2 + t3;
+    x2 = t2 - t3;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft4(cx<T>& x0, cx<T>& x1, cx<T>& x2, cx<T>& x3)
+{
+    cx<T> a = x0 + x2, b = x0 - x2, c = x1 + x3, d = mul_i<DIR>(x1 - x3);
+    x0 = a + c;
+    x2 = a - c;
+    x1 = b + d;
+    x3 = b - d;
+}
+
+// W_R^k as (cos, DIR*sin) for the constants the composite radices need
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> wk(T c, T s) { return cx<T>{c, (T)DIR * s}; }
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(cx<T>* x)
+{
+    const T h = (T)0.70710678118654752440;
+    cx<T> e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    cx<T> o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    o1 = cmul(o1, wk<DIR>(h, h));
+    o2 = mul_i<DIR>(o2);
+    o3 = cmul(o3, wk<DIR>(-h, h));
+    x[0] = e0 + o0; x[4] = e0 - o0;
+    x[1] = e1 + o1; x[5] = e1 - o1;
+    x[2] = e2 + o2; x[6] = e2 - o2;
+    x[3] = e3 + o3; x[7] = e3 - o3;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft6(cx<T>* x)
+{
+    const T s = (T)0.86602540378443864676;
+    cx<T> e0 = x[0], e1 = x[2], e2 = x[4];
+    cx<T> o0 = x[1], o1 = x[3], o2 = x[5];
+    dft3<DIR>(e0, e1, e2);
+    dft3<DIR>(o0, o1, o2);
+    o1 = cmul(o1, wk<DIR>((T)0.5, s));
+    o2 = cmul(o2, wk<DIR>((T)-0.5, s));
+    x[0] = e0 + o0; x[3] = e0 - o0;
+    x[1] = e1 + o1; x[4] = e1 - o1;
+    x[2] = e2 + o2; x[5] = e2 - o2;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft12(cx<T>* x)
+{
+    const T s = (T)0.86602540378443864676;
+    cx<T> e[6] = {x[0], x[2], x[4], x[6], x[8], x[10]};
+    cx<T> o[6] = {x[1], x[3], x[5], x[7], x[9], x[11]};
+    dft6<DIR>(e);
+    dft6<DIR>(o);
+    o[1] = cmul(o[1], wk<DIR>(s, (T)0.5));
+    o[2] = cmul(o[2], wk<DIR>((T)0.5, s));
+    o[3] = mul_i<DIR>(o[3]);
+    o[4] = cmul(o[4], wk<DIR>((T)-0.5, s));
+    o[5] = cmul(o[5], wk<DIR>(-s, (T)0.5));
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        x[k] = e[k] + o[k];
+        x[k + 6] = e[k] - o[k];
+    }
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft16(cx<T>* x)
+{
+    const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+    cx<T> e[8], o[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) { e[k] = x[2 * k]; o[k] = x[2 * k + 1]; }
+    dft8<DIR>(e);
+    dft8<DIR>(o);
+    o[1] = cmul(o[1], wk<DIR>(c1, s1));
+    o[2] = cmul(o[2], wk<DIR>(h, h));
+    o[3] = cmul(o[3], wk<DIR>(s1, c1));
+    o[4] = mul_i<DIR>(o[4]);
+    o[5] = cmul(o[5], wk<DIR>(-s1, c1));
+    o[6] = cmul(o[6], wk<DIR>(-h, h));
+    o[7] = cmul(o[7], wk<DIR>(-c1, s1));
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        x[k] = e[k] + o[k];
+        x[k + 8] = e[k] - o[k];
+    }
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft(cx<T>* x)
+{
+    if constexpr (R == 2) dft2<DIR>(x[0], x[1]);
+    else if constexpr (R == 3) dft3<DIR>(x[0], x[1], x[2]);
+    else if constexpr (R == 4) dft4<DIR>(x[0], x[1], x[2], x[3]);
+    else if constexpr (R == 6) dft6<DIR>(x);
+    else if constexpr (R == 8) dft8<DIR>(x);
+    else if constexpr (R == 12) dft12<DIR>(x);
+    else if constexpr (R == 16) dft16<DIR>(x);
+    else static_assert(R == 2, "unsupported radix");
+}
+
+// ---------------------------------------------------------------------------
+// Radix plans for the supported transform lengths (2^a 3^b, b <= 2).
+// ---------------------------------------------------------------------------
+template <int L>
+struct Plan;
+#define SPNS_PLAN(LEN, ...)                                                \
+    template <>                                                           \
+    struct Plan<LEN> {                                                    \
+        static constexpr int n = sizeof((int[]){__VA_ARGS__}) / sizeof(int); \
+        __host__ __device__ static constexpr int r(int s)                 \
+        {                                                                 \
+            constexpr int a[] = {__VA_ARGS__};                            \
+            return a[s];                                                  \
+        }                                                                 \
+    };
+SPNS_PLAN(16, 4, 4)
+SPNS_PLAN(24, 3, 8)
+SPNS_PLAN(32, 4, 8)
+SPNS_PLAN(36, 3, 12)
+SPNS_PLAN(48, 6, 8)
+SPNS_PLAN(64, 8, 8)
+SPNS_PLAN(72, 6, 12)
+SPNS_PLAN(96, 12, 8)
+SPNS_PLAN(128, 2, 8, 8)
+SPNS_PLAN(144, 12, 12)
+SPNS_PLAN(192, 3, 8, 8)
+SPNS_PLAN(256, 4, 8, 8)
+SPNS_PLAN(288, 6, 6, 8)
+SPNS_PLAN(384, 6, 8, 8)
+SPNS_PLAN(512, 8, 8, 8)
+SPNS_PLAN(576, 12, 6, 8)
+SPNS_PLAN(768, 12, 8, 8)
+SPNS_PLAN(1024, 16, 8, 8)
+SPNS_PLAN(1152, 12, 12, 8)
+SPNS_PLAN(1536, 12, 16, 8)
+#undef SPNS_PLAN
+
+template <int L, int S>
+__host__ __device__ constexpr int plan_span()  // product of the radices before stage S
+{
+    int p = 1;
+    for (int s = 0; s < S; s++) p *= Plan<L>::r(s);
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Stockham FFT through LDS.
+//   T: compute type; L: length; NB: columns; NT: threads; LD: LDS row pitch
+//   DIR: -1 forward exp(-i..), +1 inverse; SYNC0: the first stage's loader
+//   reads LDS (barrier between its reads and its writes).
+// lds must hold L*LD elements.  tw[m] = exp(-2 pi i m / L) (forward table).
+// Thread mapping: consecutive threads take consecutive columns b, so the
+// loader's global reads are coalesced when columns are contiguous.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false>
+struct Fft {
+    using C = cx<T>;
+    static constexpr int NS = Plan<L>::n;
+
+    template <int S, class Load, class Store>
+    __device__ static __forceinline__ void stage(C* __restrict__ lds, const C* __restrict__ tw, Load& load, Store& store)
+    {
+        constexpr int R = Plan<L>::r(S);
+        constexpr int Ns = plan_span<L, S>();
+        constexpr int LR = L / R;
+        constexpr int NBF = LR * NB;               // butterflies in this stage
+        constexpr int BPT = (NBF + NT - 1) / NT;   // per thread
+        constexpr int TWS = L / (Ns * R);          // twiddle index stride
+        const int tid = threadIdx.x;
+        C v[BPT][R];
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int idx = tid + q * NT;
+            if (NBF % NT == 0 || idx < NBF) {
+                const int b = idx % NB, j = idx / NB;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if constexpr (S == 0) v[q][r] = load(b, j + r * LR);
+                    else v[q][r] = lds[(j + r * LR) * LD + b];
+                }
+            }
+        }
+        if constexpr (S > 0 || SYNC0) __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int idx = tid + q * NT;
+            if (NBF % NT == 0 || idx < NBF) {
+                const int b = idx % NB, j = idx / NB;
+                const int k = j % Ns;
+                if constexpr (S > 0) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        C w = tw[r * k * TWS];
+                        if (DIR > 0) w.y = -w.y;
+                        v[q][r] = cmul(v[q][r], w);
+                    }
+                }
+                dft<R, DIR>(v[q]);
+                const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if constexpr (S == NS - 1) store(b, base + r * Ns, v[q][r]);
+                    else lds[(base + r * Ns) * LD + b] = v[q][r];
+                }
+            }
+        }
+        if constexpr (S < NS - 1) {
+            __syncthreads();
+            stage<S + 1>(lds, tw, load, store);
+        }
+    }
+
+    template <class Load, class Store>
+    __device__ static __forceinline__ void run(C* __restrict__ lds, const C* __restrict__ tw, Load load, Store store)
+    {
+        __syncthreads();  // LDS may still be read by a previous use
+        stage<0>(lds, tw, load, store);
+    }
+};
+
+}  // namespace spns

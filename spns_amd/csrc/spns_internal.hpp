@@ -1,0 +1,131 @@
+// Internal interfaces between the host context and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spns {
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+enum { PREC_FP64 = 0, PREC_FP32 = 1 };
+
+// Scalars that live in device memory so a captured step needs no host sync.
+struct DevScalars {
+    double dt, time, nu, alpha, kappa;
+    double maxvel, maxvel_sa;
+    double ek, om, relambda;
+    double skew2, skew3;
+    int64_t iter;
+    double pad[3];
+};
+
+struct StridedJob {
+    const void* in;
+    void* out;
+    int deriv;  // 1: multiply the input by i*k of the transformed dimension
+    int pad;
+};
+
+struct StridedArgs {
+    StridedJob job[9];
+    int njobs;
+    int B;     // contiguous batch width (columns)
+    int nin;   // stored input rows (odd, wrapped order) or L
+    int nout;  // stored output rows (odd, wrapped order) or L
+    int64_t in_a_stride, in_row_stride, out_a_stride, out_row_stride;  // in complex elements
+    const void* tw;
+};
+
+struct ZArgs {
+    void* f[9];  // X2 fields: u v w | ux vx wx | uy vy wy ; pencil offset pen*nk3
+    int nk3;
+    int les;
+    int64_t npencil;
+    double Ax, Ay, Az;  // CFL divisors (src/HIT_LES.h:56-58)
+    double ct;          // (C_Smag*delta)^2 (src/HIT_LES.cpp:132)
+    unsigned long long* maxvel_bits;
+    unsigned long long* maxvel_sa_bits;
+    double* skew_partial;  // [gridDim.x][2]
+    void* phys;            // MODE 1: optional [3][npencil][L] physical export
+    const void* tw;
+};
+
+struct ZPlainArgs {
+    const void* in;
+    void* out;
+    int Mx, My;
+    const void* tw;
+};
+
+enum { EPI_STEP = 0, EPI_REDUCE = 1 };
+
+struct EpiArgs {
+    const void* conv[9];  // 3 convective (+6 SGS for LES) transformed fields, compact box
+    cx<double>* uk[3];
+    cx<double>* R[3];
+    cx<double>* ukprev[3];
+    int nk1, nk2, nk3, nbins;
+    int64_t nmodes;
+    int notcubic, Lx, Ly, Lz, Mz;
+    int mode, les, rot, force_on, kfmin, kfmax, sa;
+    double nf;  // Mx*My*Mz
+    double ox, oy, oz;
+    const DevScalars* scal;
+    double* hist_partial;  // [grid][nbins]
+    double* om_partial;    // [grid]
+};
+
+enum { FIN_REDUCE = 1, FIN_SKEW = 2, FIN_MAXVEL = 4, FIN_DT = 8, FIN_ADVANCE = 16 };
+
+struct FinArgs {
+    DevScalars* scal;
+    int flags;
+    int nbins, nblk_epi, nblk_z;
+    const double* hist_partial;
+    const double* om_partial;
+    const double* skew_partial;
+    const unsigned long long* maxvel_bits;
+    const unsigned long long* maxvel_sa_bits;
+    double* ek;  // [nbins]
+    int kfmin, kfmax;
+    double force_eps;
+    double Ax, Ay, Az, C_At, fixed_dt;
+    int sa;
+    double Ah, t1, phi[3], c[14], kk[2];
+};
+
+struct ShellArgs {
+    cx<double>* uk[3];
+    int nk1, nk2, nk3;
+    int64_t nmodes;
+    int notcubic, Lx, Ly, Lz;
+    const double* ft;
+    int last_eff, nullify;
+};
+
+struct NormArgs {
+    void* data;
+    int Mx, My, Mz, Nx2, Ny2, Nz2;
+    int notcubic, Lx, Ly, Lz, all;
+    double nf;
+};
+
+bool supported_length(int L);
+int strided_tile(int L, int prec);
+int zpass_threads(int L);
+hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st);
+hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);
+hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);
+hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st);
+hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid, hipStream_t st);
+hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st);
+hipError_t launch_finalize(const FinArgs& a, hipStream_t st);
+hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st);
+hipError_t launch_forcing(const double* ek, const double* forced, double* ft, int last_eff, const ShellArgs& a, hipStream_t st);
+hipError_t launch_ccc(const ShellArgs& a, hipStream_t st);
+hipError_t launch_normalize(int prec, const NormArgs& a, hipStream_t st);
+
+}  // namespace spns

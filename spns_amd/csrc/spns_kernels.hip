@@ -1,0 +1,844 @@
+// HIP kernels for the pseudospectral HIT step on MI355X (gfx950).
+//
+// Data layout in HBM (k3 contiguous everywhere):
+//   state u_c, R_c      : FP64 compact retained box [i2][i1][i3], nk2 x nk1 x nk3,
+//                          the reference's Fourier-dump order (src/HIT_LES_output.cpp:46-53)
+//   X1 (after the y pass): T complex [y][i1][i3], My x nk1 x nk3 per field
+//   X2 (after the x pass): T complex [y][x][i3],  My x Mx  x nk3 per field
+// Passes of one step (DNS; LES adds six SGS fields to the forward passes):
+//   inv_y  : u_c -> {F_c = FFT_y u_c, G_c = FFT_y(i k2 u_c)}          (strided, pruned in)
+//   inv_x  : {F_c, G_c} -> {u_c, d_x u_c, d_y u_c}                   (strided, pruned in)
+//   z pass : c2r of 12 fields (d_z from i k3), advective products,
+//            Smagorinsky nu_t and tau, CFL max, skewness, r2c          (pencil, pruned in/out)
+//   fwd_x, fwd_y : forward transforms back to the compact box         (strided, pruned out)
+//   epilogue: normalise, R, rotation, forcing, AB2 / kappa, projection,
+//            E(K) and enstrophy partials                              (elementwise, FP64)
+//   finalize: fixed-order sums, CFL time step, time, iteration count
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "spns_device.hpp"
+#include "spns_internal.hpp"
+
+#ifndef SPNS_PART
+#define SPNS_PART 0
+#endif
+
+namespace spns {
+
+// ---------------------------------------------------------------------------
+// shell index K = round(|k|) computed exactly in integers (src/HIT_LES.cpp:1166)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int shell_of(int k2)
+{
+    int m = (int)sqrt((double)k2);
+    while (m * m > k2) m--;
+    while ((m + 1) * (m + 1) <= k2) m++;
+    return (k2 > m * m + m) ? m + 1 : m;
+}
+
+__device__ __forceinline__ int wrap_k(int i, int n)  // compact index -> wavenumber
+{
+    return (i <= n / 2) ? i : i - n;
+}
+
+// ---------------------------------------------------------------------------
+// Strided batched pass: transforms along a row dimension of [A][rows][B]
+// (B contiguous), TB consecutive columns per block.
+// ---------------------------------------------------------------------------
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR>
+__global__ __launch_bounds__(NT) void k_strided(StridedArgs a)
+{
+    __shared__ cx<T> lds[L * TB];
+    const int b0 = blockIdx.x * TB;
+    const int64_t ai = blockIdx.y;
+    const cx<T>* tw = (const cx<T>*)a.tw;
+    const int hin = a.nin / 2, hout = a.nout / 2;
+    for (int jb = 0; jb < a.njobs; jb++) {
+        const cx<TIN>* __restrict__ in = (const cx<TIN>*)a.job[jb].in + ai * a.in_a_stride;
+        cx<TOUT>* __restrict__ out = (cx<TOUT>*)a.job[jb].out + ai * a.out_a_stride;
+        const int deriv = a.job[jb].deriv;
+        auto load = [&](int b, int p) -> cx<T> {
+            const int bb = b0 + b;
+            int i, k;
+            if (a.nin == L) { i = p; k = (p <= L / 2) ? p : p - L; }
+            else if (p <= hin) { i = p; k = p; }
+            else if (p >= L - hin) { i = a.nin - (L - p); k = p - L; }
+            else return cx<T>{(T)0, (T)0};
+            if (bb >= a.B) return cx<T>{(T)0, (T)0};
+            cx<TIN> v = in[(int64_t)i * a.in_row_stride + bb];
+            if (deriv) {
+                // i k u, formed in the input precision (src/HIT_LES.cpp:37)
+                const TIN kk = (TIN)k;
+                return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
+            }
+            return cx<T>{(T)v.x, (T)v.y};
+        };
+        auto store = [&](int b, int p, cx<T> v) {
+            const int bb = b0 + b;
+            int i;
+            if (a.nout == L || p <= hout) i = p;
+            else if (p >= L - hout) i = a.nout - (L - p);
+            else return;
+            if (bb >= a.B) return;
+            out[(int64_t)i * a.out_row_stride + bb] = cx<TOUT>{(TOUT)v.x, (TOUT)v.y};
+        };
+        Fft<T, L, TB, NT, TB, DIR>::run(lds, tw, load, store);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    // fixed-order tree: deterministic for a given NT
+    const int t = threadIdx.x;
+    __syncthreads();
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* sh)
+{
+    const int t = threadIdx.x;
+    __syncthreads();
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// z pass: one (y,x) pencil per block iteration.
+// Six paired c2r transforms of length L hold the twelve physical fields
+//   (u,v) (w,ux) (uy,uz) (vx,vy) (vz,wx) (wy,wz)
+// in LDS as [z][pair]; products are formed in place, then the forward
+// transforms of (cx,cy) (cz,-) [DNS] or (cx,cy) (cz,t11) (t21,t22) (t31,t32)
+// (t33,-) [LES] are unpacked to half spectra and written over the inputs.
+// MODE 0: full step; MODE 1: velocity only (u,v,w -> CFL, optional export).
+// ---------------------------------------------------------------------------
+template <typename T, int L, int NT, int MODE>
+__global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
+{
+    constexpr int LD = 6;
+    __shared__ cx<T> lds[L * LD];
+    __shared__ double red[NT];
+    const cx<T>* tw = (const cx<T>*)a.tw;
+    const int nk = a.nk3;
+    const int t = threadIdx.x;
+    double vmax = 0.0, vmax_sa = 0.0, s2 = 0.0, s3 = 0.0;
+    const double iAx = a.Ax, iAy = a.Ay, iAz = a.Az;
+
+    for (int64_t pen = blockIdx.x; pen < a.npencil; pen += gridDim.x) {
+        const int64_t off = pen * nk;
+        // -------- inverse: build paired Hermitian spectra and transform
+        auto field = [&](int f, int k) -> cx<T> {
+            // logical field f at k3=k (0<=k<nk):
+            // 0 u,1 v,2 w | 3 ux,4 uy,5 uz | 6 vx,7 vy,8 vz | 9 wx,10 wy,11 wz
+            if (f < 3) {
+                cx<T> v = ((const cx<T>*)a.f[f])[off + k];
+                return v;
+            }
+            const int c = (f - 3) / 3, d = (f - 3) % 3;
+            if (d == 0) return ((const cx<T>*)a.f[3 + c])[off + k];
+            if (d == 1) return ((const cx<T>*)a.f[6 + c])[off + k];
+            cx<T> v = ((const cx<T>*)a.f[c])[off + k];
+            const T kk = (T)k;
+            return cx<T>{-kk * v.y, kk * v.x};
+        };
+        constexpr int NPI = (MODE == 0) ? 6 : 2;
+        auto load_inv = [&](int b, int p) -> cx<T> {
+            const int fa = 2 * b, fb = 2 * b + 1;
+            const bool hasb = (MODE == 0) || b == 0;  // velocity mode: (u,v), (w,0)
+            if (p == 0) {
+                cx<T> A = field(fa, 0);
+                T bx = hasb ? field(fb, 0).x : (T)0;
+                return cx<T>{A.x, bx};
+            }
+            if (p < nk) {
+                cx<T> A = field(fa, p);
+                cx<T> B = hasb ? field(fb, p) : cx<T>{(T)0, (T)0};
+                return cx<T>{A.x - B.y, A.y + B.x};
+            }
+            if (p > L - nk) {
+                const int k = L - p;
+                cx<T> A = field(fa, k);
+                cx<T> B = hasb ? field(fb, k) : cx<T>{(T)0, (T)0};
+                return cx<T>{A.x + B.y, B.x - A.y};
+            }
+            return cx<T>{(T)0, (T)0};
+        };
+        auto store_lds = [&](int b, int p, cx<T> v) { lds[p * LD + b] = v; };
+        Fft<T, L, NPI, NT, LD, +1>::run(lds, tw, load_inv, store_lds);
+        __syncthreads();
+
+        // -------- physical space
+        for (int n = t; n < L; n += NT) {
+            cx<T>* row = &lds[n * LD];
+            const T u = row[0].x, v = row[0].y, w = row[1].x;
+            {
+                const double au = fabs((double)u) / iAx, av = fabs((double)v) / iAy, aw = fabs((double)w) / iAz;
+                const double m = fmax(au, fmax(av, aw));  // src/HIT_LES.cpp:393
+                vmax = fmax(vmax, m);
+                vmax_sa = fmax(vmax_sa, au + av + aw);  // src/HIT_LES.cpp:377
+            }
+            if constexpr (MODE == 1) {
+                if (a.phys) {
+                    // export physical velocity as [c][y][x][z]
+                    const int64_t P = a.npencil * L;
+                    T* o = (T*)a.phys;
+                    o[pen * L + n] = u;
+                    o[P + pen * L + n] = v;
+                    o[2 * P + pen * L + n] = w;
+                }
+            } else {
+                const T ux = row[1].y, uy = row[2].x, uz = row[2].y;
+                const T vx = row[3].x, vy = row[3].y, vz = row[4].x;
+                const T wx = row[4].y, wy = row[5].x, wz = row[5].y;
+                s2 += (double)ux * (double)ux;
+                s3 += (double)ux * (double)ux * (double)ux;
+                // advective form (src/HIT_LES.cpp:118-120)
+                const T cxv = u * ux + v * uy + w * uz;
+                const T cyv = u * vx + v * vy + w * vz;
+                const T czv = u * wx + v * wy + w * wz;
+                if (a.les) {
+                    // Smagorinsky (src/HIT_LES.cpp:130-154)
+                    const T ct = (T)a.ct;
+                    T ne = (T)2.0 * ux * ux;
+                    ne += (T)2.0 * vy * vy;
+                    ne += (T)2.0 * wz * wz;
+                    ne += (uy + vx) * (uy + vx);
+                    ne += (uz + wx) * (uz + wx);
+                    ne += (vz + wy) * (vz + wy);
+                    ne = ct * sqrt(ne);
+                    const T t11 = (T)(-2.0) * ne * ux;
+                    const T t21 = (T)(-1.0) * ne * (uy + vx);
+                    const T t22 = (T)(-2.0) * ne * vy;
+                    const T t31 = (T)(-1.0) * ne * (uz + wx);
+                    const T t32 = (T)(-1.0) * ne * (vz + wy);
+                    const T t33 = (T)(-2.0) * ne * wz;
+                    row[0] = cx<T>{cxv, cyv};
+                    row[1] = cx<T>{czv, t11};
+                    row[2] = cx<T>{t21, t22};
+                    row[3] = cx<T>{t31, t32};
+                    row[4] = cx<T>{t33, (T)0};
+                } else {
+                    row[0] = cx<T>{cxv, cyv};
+                    row[1] = cx<T>{czv, (T)0};
+                }
+            }
+        }
+        if constexpr (MODE == 0) {
+            // -------- forward transforms of the products (in place in LDS)
+            auto load_lds = [&](int b, int p) -> cx<T> { return lds[p * LD + b]; };
+            if (a.les) Fft<T, L, 5, NT, LD, -1, true>::run(lds, tw, load_lds, store_lds);
+            else Fft<T, L, 2, NT, LD, -1, true>::run(lds, tw, load_lds, store_lds);
+            __syncthreads();
+            // -------- unpack pairs to half spectra, keep k3 < nk
+            const int npairs = a.les ? 5 : 2;
+            const int nfields = a.les ? 9 : 3;
+            for (int idx = t; idx < npairs * nk; idx += NT) {
+                const int q = idx / nk, k = idx % nk;
+                const cx<T> Z = lds[k * LD + q];
+                const cx<T> Zc = lds[((L - k) % L) * LD + q];
+                const T h = (T)0.5;
+                const cx<T> A = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
+                const cx<T> B = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
+                const int fa = 2 * q, fb = 2 * q + 1;
+                ((cx<T>*)a.f[fa])[off + k] = A;
+                if (fb < nfields) ((cx<T>*)a.f[fb])[off + k] = B;
+            }
+        }
+        __syncthreads();
+    }
+    // -------- block reductions -> partials / atomics
+    vmax = block_max<NT>(vmax, red);
+    vmax_sa = block_max<NT>(vmax_sa, red);
+    if (MODE == 0) {
+        s2 = block_sum<NT>(s2, red);
+        s3 = block_sum<NT>(s3, red);
+    }
+    if (t == 0) {
+        atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
+        atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
+        if (MODE == 0) {
+            a.skew_partial[2 * blockIdx.x] = s2;
+            a.skew_partial[2 * blockIdx.x + 1] = s3;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain z transforms for the FFT_package drop-in (reference layouts):
+//   c2r: complex [y][x][nk] -> real [x][y][2nk] padded (src/util.h:75-80)
+//   r2c: real [x][y][2nk] -> complex [y][x][nk]
+// Two pencils per block iteration are paired into one complex transform.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int NT, int DIR>
+__global__ __launch_bounds__(NT) void k_zplain(ZPlainArgs a)
+{
+    __shared__ cx<T> lds[L];
+    const cx<T>* tw = (const cx<T>*)a.tw;
+    const int nk = L / 2 + 1;
+    const int64_t np = (int64_t)a.Mx * a.My;
+    for (int64_t pp = blockIdx.x; 2 * pp < np; pp += gridDim.x) {
+        const int64_t p0 = 2 * pp, p1 = 2 * pp + 1;
+        const bool has1 = p1 < np;
+        // pencil index p -> (y = p / Mx, x = p % Mx)
+        const int64_t y0 = p0 / a.Mx, x0 = p0 % a.Mx, y1 = p1 / a.Mx, x1 = p1 % a.Mx;
+        if (DIR > 0) {
+            const cx<T>* in = (const cx<T>*)a.in;
+            auto ld = [&](int64_t y, int64_t x, int k) { return in[(y * a.Mx + x) * nk + k]; };
+            auto load = [&](int b, int p) -> cx<T> {
+                (void)b;
+                cx<T> A, B = cx<T>{(T)0, (T)0};
+                if (p == 0 || (p == L / 2)) {
+                    A = ld(y0, x0, p);
+                    if (has1) B = ld(y1, x1, p);
+                    return cx<T>{A.x, B.x};
+                }
+                if (p < nk) {
+                    A = ld(y0, x0, p);
+                    if (has1) B = ld(y1, x1, p);
+                    return cx<T>{A.x - B.y, A.y + B.x};
+                }
+                const int k = L - p;
+                A = ld(y0, x0, k);
+                if (has1) B = ld(y1, x1, k);
+                return cx<T>{A.x + B.y, B.x - A.y};
+            };
+            T* out = (T*)a.out;
+            auto store = [&](int b, int p, cx<T> v) {
+                (void)b;
+                out[(x0 * a.My + y0) * 2 * nk + p] = v.x;
+                if (has1) out[(x1 * a.My + y1) * 2 * nk + p] = v.y;
+            };
+            Fft<T, L, 1, NT, 1, +1>::run(lds, tw, load, store);
+        } else {
+            const T* in = (const T*)a.in;
+            auto load = [&](int b, int p) -> cx<T> {
+                (void)b;
+                T re = in[(x0 * a.My + y0) * 2 * nk + p];
+                T im = has1 ? in[(x1 * a.My + y1) * 2 * nk + p] : (T)0;
+                return cx<T>{re, im};
+            };
+            auto store = [&](int b, int p, cx<T> v) { (void)b; lds[p] = v; };
+            Fft<T, L, 1, NT, 1, -1>::run(lds, tw, load, store);
+            __syncthreads();
+            cx<T>* out = (cx<T>*)a.out;
+            for (int k = threadIdx.x; k < nk; k += NT) {
+                const cx<T> Z = lds[k], Zc = lds[(L - k) % L];
+                const T h = (T)0.5;
+                out[(y0 * a.Mx + x0) * nk + k] = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
+                if (has1) out[(y1 * a.Mx + x1) * nk + k] = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral epilogue over the compact box (FP64 state).
+// src/HIT_LES.cpp:205-242 (R), :243-278 (constant power), :295-339 (AB2 /
+// kappa), :346-359 (projection); reductions :991-1003, :1137-1214.
+// ---------------------------------------------------------------------------
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
+{
+    constexpr int NW = NT / 64;
+    extern __shared__ double hist[];  // NW * nbins
+    __shared__ double red[NT];
+    const int nb = a.nbins;
+    for (int i = threadIdx.x; i < NW * nb; i += NT) hist[i] = 0.0;
+    __syncthreads();
+    double* myh = hist + (threadIdx.x / 64) * nb;
+    const double dt = a.scal->dt, nu = a.scal->nu, alpha = a.scal->alpha;
+    const double kap = a.scal->kappa;
+    const double k05 = 1.0 / (kap + 0.5);
+    const bool first = a.scal->iter == 0;
+    double om = 0.0;
+    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3;
+    for (int64_t m = (int64_t)blockIdx.x * NT + threadIdx.x; m < a.nmodes; m += (int64_t)gridDim.x * NT) {
+        const int i3 = (int)(m % nk3);
+        const int64_t r12 = m / nk3;
+        const int i1 = (int)(r12 % nk1), i2 = (int)(r12 / nk1);
+        const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2), k3 = i3;
+        if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) continue;  // src/HIT_LES.cpp:867-871
+        const int ik2 = k1 * k1 + k2 * k2 + k3 * k3;
+        const double rad2 = (double)ik2;
+        cx<double> U[3], R0[3], R1[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) { U[c] = a.uk[c][m]; R0[c] = a.R[c][m]; }
+        if (a.mode == EPI_STEP) {
+            const double ct = nu * rad2;
+            const bool mean = !(rad2 > 0.0);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const cx<T> cv = ((const cx<T>*)a.conv[c])[m];
+                const double cvx = (double)cv.x / a.nf, cvy = (double)cv.y / a.nf;  // Fourier_Normalization
+                R1[c].x = -(mean ? 0.0 : cvx) - ct * U[c].x;
+                R1[c].y = -(mean ? 0.0 : cvy) - ct * U[c].y;
+            }
+            if (a.les) {
+                cx<double> S[6];
+#pragma unroll
+                for (int s = 0; s < 6; s++) {
+                    const cx<T> v = ((const cx<T>*)a.conv[3 + s])[m];
+                    S[s].x = (double)v.x / a.nf;
+                    S[s].y = (double)v.y / a.nf;
+                }
+                // S: 11, 21, 22, 31, 32, 33 (src/HIT_LES.cpp:179-191)
+                const double K1 = k1, K2 = k2, K3 = k3;
+                const double dxr = -K1 * S[0].y - K2 * S[1].y - K3 * S[3].y;
+                const double dxi = K1 * S[0].x + K2 * S[1].x + K3 * S[3].x;
+                const double dyr = -K1 * S[1].y - K2 * S[2].y - K3 * S[4].y;
+                const double dyi = K1 * S[1].x + K2 * S[2].x + K3 * S[4].x;
+                const double dzr = -K1 * S[3].y - K2 * S[4].y - K3 * S[5].y;
+                const double dzi = K1 * S[3].x + K2 * S[4].x + K3 * S[5].x;
+                R1[0].x -= dxr; R1[0].y -= dxi;
+                R1[1].x -= dyr; R1[1].y -= dyi;
+                R1[2].x -= dzr; R1[2].y -= dzi;
+            }
+            if (a.rot) {
+                const double ox = a.ox, oy = a.oy, oz = a.oz;
+                R1[0].x += 2 * (oz * U[1].x - oy * U[2].x); R1[0].y += 2 * (oz * U[1].y - oy * U[2].y);
+                R1[1].x += 2 * (ox * U[2].x - oz * U[0].x); R1[1].y += 2 * (ox * U[2].y - oz * U[0].y);
+                R1[2].x += 2 * (oy * U[0].x - ox * U[1].x); R1[2].y += 2 * (oy * U[0].y - ox * U[1].y);
+            }
+            if (a.force_on) {
+                const int K = shell_of(ik2);
+                if (a.kfmin <= K && K <= a.kfmax) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) { R1[c].x += alpha * U[c].x; R1[c].y += alpha * U[c].y; }
+                } else if (K == 0) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) R1[c] = cx<double>{0.0, 0.0};
+                }
+            }
+            cx<double> N[3];
+            if (a.sa) {
+                // self-adaptive kappa scheme; the oldest level at the first
+                // step is the current field
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const cx<double> Ua = first ? U[c] : a.ukprev[c][m];
+                    a.ukprev[c][m] = U[c];
+                    N[c].x = k05 * (2.0 * kap * U[c].x + dt * ((1.0 + kap) * R1[c].x - kap * R0[c].x) - (kap - 0.5) * Ua.x);
+                    N[c].y = k05 * (2.0 * kap * U[c].y + dt * ((1.0 + kap) * R1[c].y - kap * R0[c].y) - (kap - 0.5) * Ua.y);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    N[c].x = U[c].x + dt * (1.5 * R1[c].x - 0.5 * R0[c].x);
+                    N[c].y = U[c].y + dt * (1.5 * R1[c].y - 0.5 * R0[c].y);
+                }
+            }
+            {
+                const double K1 = k1, K2 = k2, K3 = k3;
+                const double axr = (rad2 > 0) ? ((K1 * N[0].x + K2 * N[1].x + K3 * N[2].x) / rad2) : 0.0;
+                const double axi = (rad2 > 0) ? ((K1 * N[0].y + K2 * N[1].y + K3 * N[2].y) / rad2) : 0.0;
+                N[0].x -= axr * K1; N[1].x -= axr * K2; N[2].x -= axr * K3;
+                N[0].y -= axi * K1; N[1].y -= axi * K2; N[2].y -= axi * K3;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                a.uk[c][m] = N[c];
+                a.R[c][m] = R1[c];
+                U[c] = N[c];
+            }
+        }
+        // reductions on the (new) state
+        const double ke = 0.5 * (U[0].x * U[0].x + U[0].y * U[0].y + U[1].x * U[1].x + U[1].y * U[1].y +
+                                 U[2].x * U[2].x + U[2].y * U[2].y);
+        const bool conj = (k3 != 0) && !((a.Mz % 2 == 0) && (k3 == a.Mz / 2));
+        const int K = shell_of(ik2);
+        atomicAdd(&myh[K], conj ? 2 * ke : ke);
+        const double en = 2 * rad2 * ke;
+        om += conj ? 2 * en : en;
+    }
+    __syncthreads();
+    for (int K = threadIdx.x; K < nb; K += NT) {
+        double s = 0.0;
+        for (int w = 0; w < NW; w++) s += hist[w * nb + K];
+        a.hist_partial[(int64_t)blockIdx.x * nb + K] = s;
+    }
+    om = block_sum<NT>(om, red);
+    if (threadIdx.x == 0) a.om_partial[blockIdx.x] = om;
+}
+
+// ---------------------------------------------------------------------------
+// Self-adaptive time step helpers (src/HIT_LES.h:249-269, src/HIT_LES.cpp:1049-1078)
+// ---------------------------------------------------------------------------
+static __device__ double FuL(double x, double x0, double x1, double f0, double f1) { return f0 + (x - x0) * (f1 - f0) / (x1 - x0); }
+static __device__ double FuQ(double x, double x0, double x1) { return (x - x0) * (x - x1); }
+static __device__ double FuG(double x, double a, double b, double c, double x0, double x1, double f0, double f1)
+{
+    return (a * x * x + b * x + c) * FuQ(x, x0, x1) + FuL(x, x0, x1, f0, f1);
+}
+
+// ---------------------------------------------------------------------------
+// Finalize: fixed-order sums of the partials; time step (src/HIT_LES.cpp:365-403)
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(NT) void k_finalize(FinArgs a)
+{
+    __shared__ double red[NT];
+    DevScalars* s = a.scal;
+    const int t = threadIdx.x;
+    if (a.flags & FIN_REDUCE) {
+        for (int K = t; K < a.nbins; K += NT) {
+            double v = 0.0;
+            for (int b = 0; b < a.nblk_epi; b++) v += a.hist_partial[(int64_t)b * a.nbins + K];
+            a.ek[K] = v;
+        }
+        double om = 0.0;
+        for (int b = t; b < a.nblk_epi; b += NT) om += a.om_partial[b];
+        om = block_sum<NT>(om, red);
+        __syncthreads();
+        if (t == 0) {
+            double e = 0.0;
+            for (int K = 0; K < a.nbins; K++) e += a.ek[K];
+            s->ek = e;
+            s->om = om;
+            s->relambda = e / s->nu * sqrt(20.0 / 3.0 / om);
+            // constant-power coefficient for the next step (src/HIT_LES.cpp:251-256)
+            double ef = 0.0;
+            for (int K = a.kfmin; K <= a.kfmax && K < a.nbins; K++) ef += a.ek[K];
+            s->alpha = (ef > 1e-30) ? a.force_eps / (2.0 * ef) : 0.0;
+        }
+    }
+    if (a.flags & FIN_SKEW) {
+        double s2 = 0.0, s3 = 0.0;
+        for (int b = t; b < a.nblk_z; b += NT) { s2 += a.skew_partial[2 * b]; s3 += a.skew_partial[2 * b + 1]; }
+        s2 = block_sum<NT>(s2, red);
+        s3 = block_sum<NT>(s3, red);
+        if (t == 0) { s->skew2 = s2; s->skew3 = s3; }
+    }
+    if (t == 0) {
+        if (a.flags & FIN_MAXVEL) {
+            s->maxvel = __longlong_as_double((long long)*a.maxvel_bits);
+            s->maxvel_sa = __longlong_as_double((long long)*a.maxvel_sa_bits);
+        }
+        if (a.flags & FIN_DT) {
+            const double nu = s->nu;
+            if (a.sa) {
+                const double MaxVel = s->maxvel_sa, Ah = a.Ah;
+                const double phi0 = atan(MaxVel * Ah * Ah / (0.35 * nu * 12.0));
+                double Topt, Kopt;
+                const double PI = 3.141592653589793238462643;
+                if (phi0 >= 0.0 && phi0 < a.phi[0]) Topt = FuG(phi0, 0.0, a.c[0], a.c[1], 0.0, a.phi[0], 4.0 / 3.0, a.t1);
+                else Topt = FuG(phi0, a.c[2], a.c[3], a.c[4], a.phi[0], PI / 2.0, a.t1, 1.0);
+                if (phi0 >= 0.0 && phi0 <= a.phi[0]) Kopt = 1.0;
+                else if (phi0 > a.phi[0] && phi0 <= a.phi[1]) Kopt = FuG(phi0, a.c[5], a.c[6], a.c[7], a.phi[0], a.phi[1], 1.0, a.kk[0]);
+                else if (phi0 > a.phi[1] && phi0 <= a.phi[2]) Kopt = FuG(phi0, a.c[8], a.c[9], a.c[10], a.phi[1], a.phi[2], a.kk[0], a.kk[1]);
+                else Kopt = FuG(phi0, a.c[11], a.c[12], a.c[13], a.phi[2], PI / 2.0, a.kk[1], 0.0);
+                double At = Topt / sqrt((MaxVel * MaxVel) / (0.35 * 0.35) + (144.0 * nu * nu / (Ah * Ah * Ah * Ah)));
+                s->kappa = Kopt;
+                s->dt = a.C_At * At;
+            } else {
+                const double ax2 = a.Ax * a.Ax, ay2 = a.Ay * a.Ay, az2 = a.Az * a.Az;
+                const double C_visc = 0.2 * fmin(ax2, fmin(ay2, az2)) / nu;
+                const double C_conv = 0.35 / s->maxvel;
+                double At = fmin(C_visc, C_conv);
+                s->dt = a.C_At * At;
+                if (a.fixed_dt > 0.0) s->dt = a.fixed_dt;
+            }
+        }
+        if (a.flags & FIN_ADVANCE) {
+            s->time = s->time + s->dt;
+            s->iter = s->iter + 1;
+        }
+    }
+}
+
+static __global__ void k_clear(unsigned long long* a, unsigned long long* b)
+{
+    if (threadIdx.x == 0) { *a = 0ull; *b = 0ull; }
+}
+
+// Spectrum imposition (src/HIT_LES.cpp:410-444): FT[K] from E(K), then scale.
+static __global__ void k_forcing_ft(const double* ek, const double* forced, double* ft, int last_eff)
+{
+    for (int K = threadIdx.x; K <= last_eff; K += blockDim.x)
+        ft[K] = ((ek[K] > 0.0) ? sqrt(forced[K] / ek[K]) : 0.0);
+}
+
+static __global__ void k_scale_shells(ShellArgs a)
+{
+    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3;
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < a.nmodes; m += (int64_t)gridDim.x * blockDim.x) {
+        const int i3 = (int)(m % nk3);
+        const int64_t r12 = m / nk3;
+        const int i1 = (int)(r12 % nk1), i2 = (int)(r12 / nk1);
+        const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2), k3 = i3;
+        if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) continue;
+        const int K = shell_of(k1 * k1 + k2 * k2 + k3 * k3);
+        for (int c = 0; c < 3; c++) {
+            cx<double> v = a.uk[c][m];
+            if (K <= a.last_eff) { v.x *= a.ft[K]; v.y *= a.ft[K]; }
+            else if (a.nullify) v = cx<double>{0.0, 0.0};
+            a.uk[c][m] = v;
+        }
+    }
+}
+
+// Complex conjugate correction of the k3 = 0 plane (src/HIT_LES.cpp:446-476)
+static __global__ void k_ccc(ShellArgs a)
+{
+    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3;
+    const int n = nk1 * nk2;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int i1 = q % nk1, i2 = q / nk1;
+        const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2);
+        if (k1 < 0 || (k1 == 0 && k2 < 0)) {
+            const int j1 = (-k1 >= 0) ? -k1 : nk1 - k1;
+            const int j2 = (-k2 >= 0) ? -k2 : nk2 - k2;
+            const int64_t m = ((int64_t)i2 * nk1 + i1) * nk3;
+            const int64_t mc = ((int64_t)j2 * nk1 + j1) * nk3;
+            for (int c = 0; c < 3; c++) {
+                cx<double> v = a.uk[c][mc];
+                a.uk[c][m] = cx<double>{v.x, -v.y};
+            }
+        }
+    }
+}
+
+// Normalisation of a reference-layout complex field on the retained set
+// (src/FFTW_package.cpp:33-42).
+template <typename T>
+__global__ void k_normalize(NormArgs a)
+{
+    const int nk = a.Mz / 2 + 1;
+    const int64_t n = (int64_t)a.My * a.Mx * nk;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int k3 = (int)(q % nk);
+        const int64_t r = q / nk;
+        const int ax = (int)(r % a.Mx), by = (int)(r / a.Mx);
+        const int k1 = (ax <= a.Mx / 2) ? ax : ax - a.Mx, k2 = (by <= a.My / 2) ? by : by - a.My;
+        bool rel = (abs(k1) <= a.Nx2 && abs(k2) <= a.Ny2 && k3 <= a.Nz2);
+        if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) rel = false;
+        if (a.all || rel) {
+            cx<T>* p = (cx<T>*)a.data;
+            p[q].x = p[q].x / (T)a.nf;
+            p[q].y = p[q].y / (T)a.nf;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch tables
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+constexpr int strided_tb()
+{
+    // largest column tile with the FFT tile in <= 48 KB of LDS
+    constexpr int bytes = (int)sizeof(cx<T>) * L;
+    return (bytes * 16 <= 49152) ? 16 : (bytes * 8 <= 49152) ? 8 : (bytes * 4 <= 49152) ? 4 : (bytes * 2 <= 49152) ? 2 : 1;
+}
+
+template <int L>
+constexpr int zpass_nt()
+{
+    return L >= 768 ? 512 : (L >= 288 ? 256 : (L >= 144 ? 128 : 64));
+}
+
+template <typename TIN, typename TOUT, typename T, int L, int DIR>
+static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
+{
+    constexpr int TB = strided_tb<T, L>();
+    constexpr int NT = 256;
+    dim3 grid((a.B + TB - 1) / TB, A, 1);
+    hipLaunchKernelGGL((k_strided<TIN, TOUT, T, L, TB, NT, DIR>), grid, dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+#define SPNS_FOR_EACH_L(X) \
+    X(16) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768) X(1024) X(1152) X(1536)
+
+#if SPNS_PART == 0 || SPNS_PART == 4
+int strided_tile(int L, int prec)
+{
+#define X(LL) \
+    if (L == LL) return prec == PREC_FP64 ? strided_tb<double, LL>() : strided_tb<float, LL>();
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return 0;
+}
+
+bool supported_length(int L)
+{
+#define X(LL) \
+    if (L == LL) return true;
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return false;
+}
+
+#endif
+
+template <typename TIN, typename TOUT, typename T, int DIR>
+static hipError_t launch_strided_T(int L, const StridedArgs& a, int A, hipStream_t st)
+{
+#define X(LL) \
+    if (L == LL) return launch_strided_L<TIN, TOUT, T, LL, DIR>(a, A, st);
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+#if SPNS_PART == 0 || SPNS_PART == 1
+hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st)
+{
+    return dir > 0 ? launch_strided_T<double, double, double, +1>(L, a, A, st)
+                   : launch_strided_T<double, double, double, -1>(L, a, A, st);
+}
+#endif
+
+#if SPNS_PART == 0 || SPNS_PART == 2
+hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st)
+{
+    if (in_fp64) {
+        return dir > 0 ? launch_strided_T<double, float, float, +1>(L, a, A, st)
+                       : launch_strided_T<double, float, float, -1>(L, a, A, st);
+    }
+    return dir > 0 ? launch_strided_T<float, float, float, +1>(L, a, A, st)
+                   : launch_strided_T<float, float, float, -1>(L, a, A, st);
+}
+#endif
+
+#if SPNS_PART == 0 || SPNS_PART == 4
+hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st)
+{
+    (void)out_fp64;
+    if (prec == PREC_FP64) return launch_strided_fp64(L, dir, a, A, st);
+    return launch_strided_fp32(L, in_fp64, dir, a, A, st);
+}
+#endif
+
+#if SPNS_PART == 0 || SPNS_PART == 3
+template <typename T, int L, int MODE>
+static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
+{
+    constexpr int NT = zpass_nt<L>();
+    hipLaunchKernelGGL((k_zpass<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st)
+{
+#define X(LL)                                                                                  \
+    if (L == LL) {                                                                             \
+        if (prec == PREC_FP64) return mode == 0 ? launch_zpass_L<double, LL, 0>(a, grid, st)   \
+                                                : launch_zpass_L<double, LL, 1>(a, grid, st);  \
+        return mode == 0 ? launch_zpass_L<float, LL, 0>(a, grid, st)                           \
+                         : launch_zpass_L<float, LL, 1>(a, grid, st);                          \
+    }
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+#endif
+
+#if SPNS_PART == 0 || SPNS_PART == 4
+template <typename T, int L, int DIR>
+static hipError_t launch_zplain_L(const ZPlainArgs& a, int grid, hipStream_t st)
+{
+    constexpr int NT = 64;
+    hipLaunchKernelGGL((k_zplain<T, L, NT, DIR>), dim3(grid), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid, hipStream_t st)
+{
+#define X(LL)                                                                                        \
+    if (L == LL) {                                                                                   \
+        if (prec == PREC_FP64) return dir > 0 ? launch_zplain_L<double, LL, +1>(a, grid, st)         \
+                                              : launch_zplain_L<double, LL, -1>(a, grid, st);        \
+        return dir > 0 ? launch_zplain_L<float, LL, +1>(a, grid, st) : launch_zplain_L<float, LL, -1>(a, grid, st); \
+    }
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+int zpass_threads(int L)
+{
+#define X(LL) \
+    if (L == LL) return zpass_nt<LL>();
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return 0;
+}
+
+hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st)
+{
+    constexpr int NT = 256;
+    const size_t shm = sizeof(double) * (NT / 64) * a.nbins;
+    if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue<double, NT>), dim3(grid), dim3(NT), shm, st, a);
+    else hipLaunchKernelGGL((k_epilogue<float, NT>), dim3(grid), dim3(NT), shm, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const FinArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_finalize<256>), dim3(1), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_forcing(const double* ek, const double* forced, double* ft, int last_eff, const ShellArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_forcing_ft, dim3(1), dim3(256), 0, st, ek, forced, ft, last_eff);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int grid = (int)((a.nmodes + 255) / 256);
+    if (grid > 4096) grid = 4096;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_scale_shells, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ccc(const ShellArgs& a, hipStream_t st)
+{
+    int n = a.nk1 * a.nk2;
+    int grid = (n + 255) / 256;
+    hipLaunchKernelGGL(k_ccc, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize(int prec, const NormArgs& a, hipStream_t st)
+{
+    const int64_t n = (int64_t)a.My * a.Mx * (a.Mz / 2 + 1);
+    int grid = (int)((n + 255) / 256);
+    if (grid > 8192) grid = 8192;
+    if (prec == PREC_FP64) hipLaunchKernelGGL((k_normalize<double>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_normalize<float>), dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+#endif
+}  // namespace spns
