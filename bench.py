@@ -1,0 +1,241 @@
+"""Benchmark: RK (AB2) steps/s and HBM roofline at 512^3, FP32-split vs FP64.
+
+Workload (the 512^3 forced DNS): Nx=Ny=Nz=512 retained
+modes, FFT grid 768^3, nu=2e-4, constant-power forcing eps=0.1 on shells
+1<=K<=2, CFL step with C_At=0.5, seeded random solenoidal field with
+E(k) ~ k^4 exp(-2 (k/k0)^2), k0=4, u_rms=1 (synthetic data).
+
+Timed region: K captured steps (hipGraph) with the state resident in HBM,
+bracketed by stream and device synchronisation.  `value` is the FP32-split
+form's steps/s; the FP64 plain form is measured in the same run.
+
+The roofline object covers the dominant kernel of the FP32-split step:
+achieved = its algorithmic bytes per launch / its mean launch time, the
+time from HIP events on the library's stream (spns_profile_passes).
+`traffic` comes from the committed rocprofv3 PMC summary of the same
+kernel and workload when present (profiles/*.json), else null.
+
+cpu_baseline: the FP64 C oracle (a port of the reference step) on the
+host's cores, one step of the same forced-DNS setup at a bounded size.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 512]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E bytes/s
+METRIC = "RK steps/s + HBM % at 512^3, FP32-split vs FP64; ensemble p50/p99 latency"
+
+
+def dims(N):
+    M = (3 * N) // 2
+    nk1 = nk2 = 2 * (N // 2) + 1
+    nk3 = N // 2 + 1
+    return M, nk1, nk2, nk3
+
+
+def pass_bytes(N, prec, les=False):
+    """Algorithmic (compulsory) HBM bytes of each pass of one step."""
+    M, nk1, nk2, nk3 = dims(N)
+    sc = 16 if prec == 0 else 8  # complex of the transform precision
+    KN = nk1 * nk2 * nk3
+    x1 = M * nk1 * nk3  # one field after the y pass
+    x2 = M * M * nk3    # one field after the x pass
+    nf = 9 if les else 3
+    return {
+        "inv_y": 3 * KN * 16 + 6 * x1 * sc,
+        "inv_x": 6 * x1 * sc + 9 * x2 * sc,
+        "zpass": 9 * x2 * sc + nf * x2 * sc,
+        "fwd_x": nf * x2 * sc + nf * x1 * sc,
+        "fwd_y": nf * x1 * sc + nf * KN * sc,
+        "epilogue": nf * KN * sc + 6 * KN * 16 + 6 * KN * 16,
+    }
+
+
+def model_bytes(N, prec, les=False):
+    """BASELINE.md's model: n_fft * 2 * s_t * M^3 + 12 * 16 * K_N."""
+    M, nk1, nk2, nk3 = dims(N)
+    n_fft = 21 if les else 15
+    s_t = 8 if prec == 0 else 4
+    return n_fft * 2 * s_t * M ** 3 + 12 * 16 * nk1 * nk2 * nk3
+
+
+def sync_all(solver):
+    solver.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def run_form(N, prec, steps, warmup, seed, prof_steps):
+    import spns_amd
+    s = spns_amd.Solver(Nx=N, nu=2e-4, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2,
+                        precision=prec, use_graph=True)
+    s.init_random(seed=seed, k0=4.0, urms=1.0)
+    s.finish_init(False)
+    s.step(warmup)
+    sync_all(s)
+    t0 = time.perf_counter()
+    s.step(steps)
+    sync_all(s)
+    t1 = time.perf_counter()
+    ev_ms = s.last_step_ms()
+    sc = s.scalars()
+    ek = s.spectrum()
+    passes = s.profile_passes(prof_steps) if prof_steps > 0 else {}
+    s.close()
+    return {"wall_s": t1 - t0, "event_ms": ev_ms, "scalars": sc, "ek": ek, "passes": passes}
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_%s_%s.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch")
+
+
+def cpu_baseline(n_cpu, threads):
+    """One step of the same forced-DNS setup at N=n_cpu on the host (oracle)."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle.oracle import Oracle
+    import numpy as np
+    o = Oracle(n_cpu, nu=2e-4, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2)
+    rng = np.random.default_rng(0)
+    st = (rng.standard_normal((3, o.nmodes)) + 1j * rng.standard_normal((3, o.nmodes))) * 1e-3
+    o.set_state(st)
+    o.finish_init(False)
+    t0 = time.perf_counter()
+    o.step(1)
+    dt = time.perf_counter() - t0
+    from oracle.oracle import lib
+    return dt, lib().orc_num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--prof-steps", type=int, default=3)
+    ap.add_argument("--cpu-N", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,fp32")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs one GPU; the step shards as a slab decomposition only in design "
+              "(see DESIGN.md).  Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    N = a.N
+    M = dims(N)[0]
+    res = {}
+    forms = [f.strip() for f in a.forms.split(",") if f.strip()]
+    for form in forms:
+        prec = 0 if form == "fp64" else 1
+        print("bench: %s N=%d (M=%d) warmup=%d steps=%d" % (form, N, M, a.warmup, a.steps), file=sys.stderr, flush=True)
+        res[form] = run_form(N, prec, a.steps, a.warmup, a.seed, a.prof_steps)
+        r = res[form]
+        print("bench: %s %.3f ms/step (events %.3f ms over %d steps)" % (form, 1e3 * r["wall_s"] / a.steps,
+              r["event_ms"] / a.steps, a.steps), file=sys.stderr, flush=True)
+
+    main_form = "fp32" if "fp32" in res else forms[0]
+    r = res[main_form]
+    ms = 1e3 * r["wall_s"] / a.steps
+    sps = a.steps / r["wall_s"]
+    prec_main = 1 if main_form == "fp32" else 0
+
+    # dominant kernel roofline from per-pass HIP-event timings
+    pb = pass_bytes(N, prec_main)
+    passes = r["passes"]
+    roof = None
+    if passes:
+        dom = max((k for k in pb if k in passes), key=lambda k: passes[k])
+        ach = pb[dom] / (passes[dom] * 1e-3)
+        wl = "N%d_%s" % (N, main_form)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": ach / PEAK_HBM, "traffic": load_traffic(dom, wl), "algorithmic_bytes": pb[dom],
+                "launch_ms": passes[dom]}
+
+    forms_out = {}
+    for form, rr in res.items():
+        prec = 0 if form == "fp64" else 1
+        sp = a.steps / rr["wall_s"]
+        forms_out[form] = {
+            "steps_per_s": sp,
+            "ms_per_step": 1e3 * rr["wall_s"] / a.steps,
+            "grid_points_steps_per_s": sp * M ** 3,
+            "hbm_pct_model": 100.0 * model_bytes(N, prec) * sp / PEAK_HBM,
+            "bytes_per_step_algorithmic": sum(pass_bytes(N, prec).values()),
+            "pass_ms": rr["passes"],
+            "ek_total": rr["scalars"]["ek"],
+            "epsilon": rr["scalars"]["nu"] * rr["scalars"]["om"],
+            "skewness": rr["scalars"]["skewness"],
+        }
+    split_err = None
+    if "fp64" in res and "fp32" in res:
+        import numpy as np
+        e64, e32 = res["fp64"]["ek"], res["fp32"]["ek"]
+        s64, s32 = res["fp64"]["scalars"], res["fp32"]["scalars"]
+        split_err = {
+            "steps": a.warmup + a.steps,
+            "ek_rel_l2": float(np.linalg.norm(e32 - e64) / np.linalg.norm(e64)),
+            "epsilon_rel": abs(s32["nu"] * s32["om"] - s64["nu"] * s64["om"]) / (s64["nu"] * s64["om"]),
+            "skewness_abs": abs(s32["skewness"] - s64["skewness"]),
+            "reference": "FP64 plain GPU form (which matches the FP64 oracle to 1e-12 at test sizes)",
+        }
+
+    cpu = None
+    if not a.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        print("bench: cpu baseline, oracle 1 step at N=%d" % a.cpu_N, file=sys.stderr, flush=True)
+        dt, used = cpu_baseline(a.cpu_N, threads)
+        Mc = dims(a.cpu_N)[0]
+        scale = (M ** 3 * math.log2(M ** 3)) / (Mc ** 3 * math.log2(Mc ** 3))
+        cpu = {"value": 1.0 / (dt * scale), "unit": "steps/s", "cores": used, "kind": "port",
+               "sample": "1 step of the same forced-DNS setup at N=%d (FFT grid %d^3) took %.2f s on %d threads; "
+                         "value scaled to N=%d by the P*log2(P) ratio %.2f" % (a.cpu_N, Mc, dt, used, N, scale),
+               "measured_steps_per_s_at_sample": 1.0 / dt}
+
+    line = {
+        "metric": METRIC,
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64 split" if main_form == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "forced DNS 512^3 retained modes (FFT grid %d^3), nu=2e-4, constant power 0.1 "
+                               "on 1<=K<=2, CFL C_At=0.5, k0=4 u_rms=1 seed %d" % (M, a.seed),
+                   "N": N, "M": M, "form": main_form, "parallelism": "1 GPU"},
+        "forms": forms_out,
+        "fp32_split_speedup": (forms_out["fp32"]["steps_per_s"] / forms_out["fp64"]["steps_per_s"]
+                               if "fp32" in forms_out and "fp64" in forms_out else None),
+        "fp32_split_error": split_err,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -130,10 +130,18 @@ int spns_get_scalars(spns_ctx *ctx, spns_scalars *out);
 /* HIT::Recalculate_Velocity_Antitransform + RealVelocity_to_BinaryFile's
  * data (src/HIT_LES.cpp:60-69): u,v,w as [3][Mx][My][Mz] doubles. */
 int spns_velocity_physical(spns_ctx *ctx, double *uvw);
+/* HIT::Recalculate_Velocity_Antitransform (src/HIT_LES.cpp:60-69) without
+ * copying the field out: refreshes the CFL maxima from the current state. */
+int spns_recalculate_velocity_antitransform(spns_ctx *ctx);
 /* Wait for the context's stream. */
 int spns_synchronize(spns_ctx *ctx);
 /* Device time (ms) of the last spns_step call's kernels, from HIP events. */
 int spns_last_step_ms(spns_ctx *ctx, double *ms);
+/* Runs nsteps steps launched pass by pass with HIP events between the passes
+ * on the context's stream and returns the mean ms of each pass in ms[0..7]:
+ * clear, inv_y, inv_x, z pass, fwd_x, fwd_y, epilogue, finalize.  The steps
+ * advance the solution like spns_step (no reference counterpart). */
+int spns_profile_passes(spns_ctx *ctx, int32_t nsteps, double *ms);
 
 /* ---- FFT_package drop-in (src/FFTW_package.h:21-71) on device buffers ----
  * complex layout [y][x][Mz/2+1] (FFTW_MPI_TRANSPOSED, one rank),

@@ -39,6 +39,7 @@
 #endif
 
 #define ORC_PI 3.141592653589793238462643 /* src/util.h:57 */
+#define PB 16 /* pencils gathered together in the strided passes */
 
 /* ------------------------------------------------------------------------ */
 /* Twiddles: exp(-2 pi i m/n) with octant reduction in long double so that   */
@@ -141,7 +142,7 @@ typedef struct {
     int Mx, My, Mz, Mz2;
     plan1d px, py, pz;
     int nthreads;
-    double *scratch; /* per-thread: 4 * max(M) doubles */
+    double *scratch; /* per-thread: 2 (PB+1) max(M) doubles */
     int maxM;
 } fft3d;
 
@@ -156,7 +157,7 @@ static int fft3d_init(fft3d *f, int Mx, int My, int Mz)
     f->nthreads = nt;
     f->maxM = Mx > My ? Mx : My;
     if (Mz > f->maxM) f->maxM = Mz;
-    f->scratch = (double *)malloc(sizeof(double) * 4 * (size_t)f->maxM * nt);
+    f->scratch = (double *)malloc(sizeof(double) * 2 * (PB + 1) * (size_t)f->maxM * nt);
     return 0;
 }
 
@@ -172,43 +173,62 @@ static double *thread_scratch(const fft3d *f)
 #ifdef _OPENMP
     t = omp_get_thread_num();
 #endif
-    return f->scratch + (size_t)4 * f->maxM * t;
+    return f->scratch + (size_t)2 * (PB + 1) * f->maxM * t;
 }
 
-/* in-place complex transform along x and y of a [My][Mx][Mz2+1] array */
+/* in-place complex transform along x and y of a [My][Mx][Mz2+1] array.
+ * Pencils are gathered in groups of up to PB consecutive k3 (or x) so each
+ * read touches whole cache lines. */
 static void fft_xy(const fft3d *f, double *c, int sign)
 {
     const int Mx = f->Mx, My = f->My, nz = f->Mz2 + 1;
-    /* x: for each (b,k3), pencil over a, stride nz */
+    const int ngz = (nz + PB - 1) / PB;
+    /* x: for each (b, k3 group), pencils over a, stride nz */
 #pragma omp parallel for collapse(2) schedule(static)
     for (int b = 0; b < My; b++)
-        for (int k3 = 0; k3 < nz; k3++) {
-            double *buf = thread_scratch(f), *w = buf + 2 * f->maxM;
+        for (int g = 0; g < ngz; g++) {
+            double *buf = thread_scratch(f), *w = buf + 2 * (size_t)f->maxM * PB;
+            const int k0 = g * PB, nb = (nz - k0 < PB) ? nz - k0 : PB;
             for (int a = 0; a < Mx; a++) {
-                size_t id = ((size_t)b * Mx + a) * nz + k3;
-                buf[2 * a] = c[2 * id]; buf[2 * a + 1] = c[2 * id + 1];
+                const double *src = c + 2 * (((size_t)b * Mx + a) * nz + k0);
+                for (int q = 0; q < nb; q++) {
+                    buf[2 * ((size_t)q * Mx + a)] = src[2 * q];
+                    buf[2 * ((size_t)q * Mx + a) + 1] = src[2 * q + 1];
+                }
             }
-            fft1d(&f->px, buf, w, sign);
+            for (int q = 0; q < nb; q++) fft1d(&f->px, buf + 2 * (size_t)q * Mx, w, sign);
             for (int a = 0; a < Mx; a++) {
-                size_t id = ((size_t)b * Mx + a) * nz + k3;
-                c[2 * id] = buf[2 * a]; c[2 * id + 1] = buf[2 * a + 1];
+                double *dst = c + 2 * (((size_t)b * Mx + a) * nz + k0);
+                for (int q = 0; q < nb; q++) {
+                    dst[2 * q] = buf[2 * ((size_t)q * Mx + a)];
+                    dst[2 * q + 1] = buf[2 * ((size_t)q * Mx + a) + 1];
+                }
             }
         }
-    /* y: for each (a,k3), pencil over b, stride Mx*nz */
-#pragma omp parallel for collapse(2) schedule(static)
-    for (int a = 0; a < Mx; a++)
-        for (int k3 = 0; k3 < nz; k3++) {
-            double *buf = thread_scratch(f), *w = buf + 2 * f->maxM;
-            for (int b = 0; b < My; b++) {
-                size_t id = ((size_t)b * Mx + a) * nz + k3;
-                buf[2 * b] = c[2 * id]; buf[2 * b + 1] = c[2 * id + 1];
-            }
-            fft1d(&f->py, buf, w, sign);
-            for (int b = 0; b < My; b++) {
-                size_t id = ((size_t)b * Mx + a) * nz + k3;
-                c[2 * id] = buf[2 * b]; c[2 * id + 1] = buf[2 * b + 1];
+    /* y: for each (a,k3) column group, pencils over b, stride Mx*nz */
+    const size_t row = (size_t)Mx * nz;
+    const size_t ncol = row, ngc = (ncol + PB - 1) / PB;
+#pragma omp parallel for schedule(static)
+    for (size_t g = 0; g < ngc; g++) {
+        double *buf = thread_scratch(f), *w = buf + 2 * (size_t)f->maxM * PB;
+        const size_t c0 = g * PB;
+        const int nb = (int)((ncol - c0 < PB) ? ncol - c0 : PB);
+        for (int b = 0; b < My; b++) {
+            const double *src = c + 2 * ((size_t)b * row + c0);
+            for (int q = 0; q < nb; q++) {
+                buf[2 * ((size_t)q * My + b)] = src[2 * q];
+                buf[2 * ((size_t)q * My + b) + 1] = src[2 * q + 1];
             }
         }
+        for (int q = 0; q < nb; q++) fft1d(&f->py, buf + 2 * (size_t)q * My, w, sign);
+        for (int b = 0; b < My; b++) {
+            double *dst = c + 2 * ((size_t)b * row + c0);
+            for (int q = 0; q < nb; q++) {
+                dst[2 * q] = buf[2 * ((size_t)q * My + b)];
+                dst[2 * q + 1] = buf[2 * ((size_t)q * My + b) + 1];
+            }
+        }
+    }
 }
 
 /* c2r: complex cin [My][Mx][nz] -> real rout [Mx][My][2nz]; cin preserved.
@@ -222,7 +242,7 @@ static void c2r3d(const fft3d *f, const double *cin, double *rout, double *work)
 #pragma omp parallel for collapse(2) schedule(static)
     for (int b = 0; b < My; b++)
         for (int a = 0; a < Mx; a++) {
-            double *buf = thread_scratch(f), *w = buf + 2 * f->maxM;
+            double *buf = thread_scratch(f), *w = buf + 2 * (size_t)f->maxM * PB;
             const double *h = work + 2 * (((size_t)b * Mx + a) * nz);
             for (int k = 0; k < Mz; k++) { buf[2 * k] = 0.0; buf[2 * k + 1] = 0.0; }
             for (int k = 0; k < nz; k++) {
@@ -248,7 +268,7 @@ static void r2c3d(const fft3d *f, const double *rin, double *cout)
 #pragma omp parallel for collapse(2) schedule(static)
     for (int i = 0; i < Mx; i++)
         for (int j = 0; j < My; j++) {
-            double *buf = thread_scratch(f), *w = buf + 2 * f->maxM;
+            double *buf = thread_scratch(f), *w = buf + 2 * (size_t)f->maxM * PB;
             const double *x = rin + ((size_t)i * My + j) * (2 * (size_t)nz);
             for (int k = 0; k < Mz; k++) { buf[2 * k] = x[k]; buf[2 * k + 1] = 0.0; }
             fft1d(&f->pz, buf, w, -1);

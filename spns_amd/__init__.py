@@ -53,7 +53,7 @@ EXPORTS = [
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
-    "spns_velocity_physical", "spns_synchronize", "spns_last_step_ms",
+    "spns_velocity_physical", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_last_step_ms", "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
 ]
@@ -92,8 +92,10 @@ def load_library(path=LIB_PATH):
         "spns_energy_cascade": (I32, [P, D, I32, D]),
         "spns_get_scalars": (I32, [P, ctypes.POINTER(Scalars)]),
         "spns_velocity_physical": (I32, [P, D]),
+        "spns_recalculate_velocity_antitransform": (I32, [P]),
         "spns_synchronize": (I32, [P]),
         "spns_last_step_ms": (I32, [P, D]),
+        "spns_profile_passes": (I32, [P, I32, D]),
         "spns_fft_create": (I32, [I32, I32, I32, I32, I32, ctypes.POINTER(P)]),
         "spns_fft_destroy": (I32, [P]),
         "spns_fft_r2c": (I32, [P, P, P]),
@@ -219,8 +221,20 @@ class Solver:
         self._chk(self.L.spns_last_step_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
+    PASSES = ["clear", "inv_y", "inv_x", "zpass", "fwd_x", "fwd_y", "epilogue", "finalize"]
+
+    def profile_passes(self, nsteps):
+        """Mean device ms of each pass over nsteps eagerly launched steps."""
+        ms = np.zeros(8)
+        self._chk(self.L.spns_profile_passes(self.h, nsteps, _dp(ms)))
+        return dict(zip(self.PASSES, ms.tolist()))
+
     def recalc_timestep(self):
         self._chk(self.L.spns_recalculate_timestep(self.h))
+
+    def antitransform(self):
+        """Refresh the CFL maxima from the current state (no copy-out)."""
+        self._chk(self.L.spns_recalculate_velocity_antitransform(self.h))
 
     def set_nu(self, nu):
         self._chk(self.L.spns_set_nu(self.h, nu))

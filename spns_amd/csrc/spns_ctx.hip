@@ -138,7 +138,9 @@ struct spns_ctx {
     int Nx, Ny, Nz, Mx, My, Mz, Nx_2, Ny_2, Nz_2;
     int nk1, nk2, nk3, nbins, last_rad, last_rad_max, prec, les, rot, sa;
     int ActualMx, ActualMy, ActualMz;
-    int64_t nmodes;
+    int64_t nmodes;    // compact box (C ABI layout)
+    int nk3p;          // k3 pitch of every device buffer (cache-line multiple)
+    int64_t nmodes_p;  // nk2 * nk1 * nk3p
     double Ax, Ay, Az, delta, ct, nf;
     size_t tsz;  // bytes of one complex T
     hipStream_t st = nullptr;
@@ -161,6 +163,7 @@ struct spns_ctx {
     double* forced = nullptr;
     double* ft = nullptr;
     int grid_epi = 0, grid_z = 0;
+    int zkernel = 0;  // z-pass variant (SPNS_ZPASS=block selects the block kernel)
     hipGraphExec_t graph = nullptr;
     bool graph_ready = false;
     std::string err;
@@ -169,7 +172,7 @@ struct spns_ctx {
     char* xbase(void* p, int slot, int64_t slot_elems) const { return (char*)p + (size_t)slot * slot_elems * tsz; }
     void* X1f(int f) const { return xbase(X1, f, x1_slot); }
     void* X2f(int f) const { return xbase(X2, f, x2_slot); }
-    void* Cf(int f) const { return xbase(X2, f, nmodes); }  // compact transformed fields, in X2's space
+    void* Cf(int f) const { return xbase(X2, f, nmodes_p); }  // compact transformed fields, in X2's space
 
     void init(const spns_config& c)
     {
@@ -192,6 +195,8 @@ struct spns_ctx {
         if (Nx_2 >= (Mx + 1) / 2 || Ny_2 >= (My + 1) / 2 || Nz_2 >= Mz / 2)
             throw Error(SPNS_EINVAL, "retained modes do not fit the FFT grid");
         nmodes = (int64_t)nk2 * nk1 * nk3;
+        nk3p = (nk3 + 15) / 16 * 16;
+        nmodes_p = (int64_t)nk2 * nk1 * nk3p;
         // src/HIT_LES.h:31-58
         ActualMx = (Mx % 2 == 0) ? Mx / cfg.Lx : (Mx - 1) / cfg.Lx + 1;
         ActualMy = (My % 2 == 0) ? My / cfg.Ly : (My - 1) / cfg.Ly + 1;
@@ -211,18 +216,19 @@ struct spns_ctx {
         sa = cfg.isSelfAdaptive != 0;
         nfwd = les ? 9 : 3;
 
+        if (const char* zk = std::getenv("SPNS_ZPASS")) zkernel = std::string(zk) == "block" ? 1 : 0;
         HIPCHK(hipSetDevice(cfg.device));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
-        const size_t sb = sizeof(cx<double>) * (size_t)nmodes;
+        const size_t sb = sizeof(cx<double>) * (size_t)nmodes_p;
         for (int c = 0; c < 3; c++) {
             alloc((void**)&uk[c], sb);
             alloc((void**)&R[c], sb);
             if (sa) alloc((void**)&ukprev[c], sb);
         }
-        x1_slot = (int64_t)My * nk1 * nk3;
-        x2_slot = (int64_t)My * Mx * nk3;
+        x1_slot = (int64_t)My * nk1 * nk3p;
+        x2_slot = (int64_t)My * Mx * nk3p;
         n_x1 = std::max(6, nfwd);
         alloc(&X1, (size_t)n_x1 * x1_slot * tsz);
         alloc(&X2, (size_t)9 * x2_slot * tsz);
@@ -230,7 +236,7 @@ struct spns_ctx {
         twy = upload_twiddles(My, prec);
         twz = upload_twiddles(Mz, prec);
         alloc((void**)&scal, sizeof(DevScalars));
-        grid_epi = (int)std::min<int64_t>((nmodes + 255) / 256, 2048);
+        grid_epi = (int)std::min<int64_t>((nmodes_p + 255) / 256, 1024);
         const int64_t npen = (int64_t)My * Mx;
         grid_z = (int)std::min<int64_t>(npen, 65536);
         alloc((void**)&hist_partial, sizeof(double) * (size_t)grid_epi * nbins);
@@ -240,6 +246,8 @@ struct spns_ctx {
         alloc((void**)&ek, sizeof(double) * nbins);
         alloc((void**)&ft, sizeof(double) * (nbins + 1));
         HIPCHK(hipMemsetAsync(skew_partial, 0, sizeof(double) * 2 * grid_z, st));
+        HIPCHK(hipMemsetAsync(X1, 0, (size_t)n_x1 * x1_slot * tsz, st));
+        HIPCHK(hipMemsetAsync(X2, 0, (size_t)9 * x2_slot * tsz, st));
         HIPCHK(hipMemsetAsync(mv, 0, sizeof(unsigned long long) * 2, st));
         for (int c = 0; c < 3; c++) {
             HIPCHK(hipMemsetAsync(uk[c], 0, sb, st));
@@ -275,6 +283,7 @@ struct spns_ctx {
     }
 
     // ---------------------------------------------------------------- passes
+    // y passes: rows = y (or i2), batch = (i1 outer, i3 contiguous)
     void inv_y(bool velocity_only)
     {
         StridedArgs a{};
@@ -284,15 +293,17 @@ struct spns_ctx {
             if (!velocity_only) a.job[n++] = StridedJob{uk[c], X1f(3 + c), 1, 0};
         }
         a.njobs = n;
-        a.B = nk1 * nk3; a.nin = nk2; a.nout = My;
-        a.in_row_stride = (int64_t)nk1 * nk3; a.out_row_stride = (int64_t)nk1 * nk3;
+        a.B = nk3; a.nin = nk2; a.nout = My;
+        a.in_row_stride = (int64_t)nk1 * nk3p; a.out_row_stride = (int64_t)nk1 * nk3p;
+        a.in_a_stride = nk3p; a.out_a_stride = nk3p;
         a.tw = twy;
         int64_t icap[9], ocap[9];
-        for (int j = 0; j < n; j++) { icap[j] = nmodes; ocap[j] = x1_slot; }
-        check_strided(a, 1, icap, ocap);
-        HIPCHK(launch_strided(My, prec, 1, prec == PREC_FP64, +1, a, 1, st));
+        for (int j = 0; j < n; j++) { icap[j] = nmodes_p; ocap[j] = x1_slot; }
+        check_strided(a, nk1, icap, ocap);
+        HIPCHK(launch_strided(My, prec, 1, prec == PREC_FP64, +1, a, nk1, st));
     }
 
+    // x passes: rows = x (or i1), batch = (y outer, i3 contiguous)
     void inv_x(bool velocity_only)
     {
         StridedArgs a{};
@@ -306,8 +317,8 @@ struct spns_ctx {
         }
         a.njobs = n;
         a.B = nk3; a.nin = nk1; a.nout = Mx;
-        a.in_row_stride = nk3; a.in_a_stride = (int64_t)nk1 * nk3;
-        a.out_row_stride = nk3; a.out_a_stride = (int64_t)Mx * nk3;
+        a.in_row_stride = nk3p; a.in_a_stride = (int64_t)nk1 * nk3p;
+        a.out_row_stride = nk3p; a.out_a_stride = (int64_t)Mx * nk3p;
         a.tw = twx;
         int64_t icap[9], ocap[9];
         for (int j = 0; j < n; j++) { icap[j] = x1_slot; ocap[j] = x2_slot; }
@@ -320,7 +331,10 @@ struct spns_ctx {
         ZArgs z{};
         for (int f = 0; f < 9; f++) z.f[f] = X2f(f);
         z.nk3 = nk3;
+        z.pitch = nk3p;
         z.les = les;
+        z.sa = sa;
+        z.kernel = zkernel;
         z.npencil = (int64_t)My * Mx;
         z.Ax = Ax; z.Ay = Ay; z.Az = Az;
         z.ct = ct;
@@ -329,7 +343,7 @@ struct spns_ctx {
         z.skew_partial = skew_partial;
         z.phys = phys;
         z.tw = twz;
-        if (z.npencil * nk3 > x2_slot) throw Error(SPNS_EINVAL, "z pass exceeds its buffer");
+        if (z.npencil * nk3p > x2_slot || nk3 > nk3p) throw Error(SPNS_EINVAL, "z pass exceeds its buffer");
         HIPCHK(launch_zpass(Mz, prec, mode, z, grid_z, st));
     }
 
@@ -339,8 +353,8 @@ struct spns_ctx {
         for (int f = 0; f < nfwd; f++) a.job[f] = StridedJob{X2f(f), X1f(f), 0, 0};
         a.njobs = nfwd;
         a.B = nk3; a.nin = Mx; a.nout = nk1;
-        a.in_row_stride = nk3; a.in_a_stride = (int64_t)Mx * nk3;
-        a.out_row_stride = nk3; a.out_a_stride = (int64_t)nk1 * nk3;
+        a.in_row_stride = nk3p; a.in_a_stride = (int64_t)Mx * nk3p;
+        a.out_row_stride = nk3p; a.out_a_stride = (int64_t)nk1 * nk3p;
         a.tw = twx;
         int64_t icap[9], ocap[9];
         for (int j = 0; j < nfwd; j++) { icap[j] = x2_slot; ocap[j] = x1_slot; }
@@ -353,14 +367,15 @@ struct spns_ctx {
         StridedArgs a{};
         for (int f = 0; f < nfwd; f++) a.job[f] = StridedJob{X1f(f), Cf(f), 0, 0};
         a.njobs = nfwd;
-        a.B = nk1 * nk3; a.nin = My; a.nout = nk2;
-        a.in_row_stride = (int64_t)nk1 * nk3; a.out_row_stride = (int64_t)nk1 * nk3;
+        a.B = nk3; a.nin = My; a.nout = nk2;
+        a.in_row_stride = (int64_t)nk1 * nk3p; a.out_row_stride = (int64_t)nk1 * nk3p;
+        a.in_a_stride = nk3p; a.out_a_stride = nk3p;
         a.tw = twy;
         int64_t icap[9], ocap[9];
-        for (int j = 0; j < nfwd; j++) { icap[j] = x1_slot; ocap[j] = nmodes; }
-        check_strided(a, 1, icap, ocap);
-        if ((int64_t)nfwd * nmodes > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
-        HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, 1, st));
+        for (int j = 0; j < nfwd; j++) { icap[j] = x1_slot; ocap[j] = nmodes_p; }
+        check_strided(a, nk1, icap, ocap);
+        if ((int64_t)nfwd * nmodes_p > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
+        HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, nk1, st));
     }
 
     EpiArgs epi_args(int mode)
@@ -368,8 +383,8 @@ struct spns_ctx {
         EpiArgs e{};
         for (int f = 0; f < nfwd; f++) e.conv[f] = Cf(f);
         for (int c = 0; c < 3; c++) { e.uk[c] = uk[c]; e.R[c] = R[c]; e.ukprev[c] = ukprev[c]; }
-        e.nk1 = nk1; e.nk2 = nk2; e.nk3 = nk3; e.nbins = nbins;
-        e.nmodes = nmodes;
+        e.nk1 = nk1; e.nk2 = nk2; e.nk3 = nk3; e.nk3p = nk3p; e.nbins = nbins;
+        e.nmodes = nmodes_p;
         e.notcubic = cfg.isNotCubic; e.Lx = cfg.Lx; e.Ly = cfg.Ly; e.Lz = cfg.Lz; e.Mz = Mz;
         e.mode = mode; e.les = les; e.rot = rot; e.force_on = cfg.force_on;
         e.kfmin = cfg.force_kmin; e.kfmax = cfg.force_kmax; e.sa = sa;
@@ -487,15 +502,16 @@ struct spns_ctx {
             }
         }
         for (int c = 0; c < 3; c++)
-            HIPCHK(hipMemcpyAsync(uk[c], m.data() + c * nmodes, sizeof(cx<double>) * nmodes, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpy2DAsync(uk[c], sizeof(cx<double>) * nk3p, m.data() + c * nmodes, sizeof(cx<double>) * nk3,
+                                    sizeof(cx<double>) * nk3, (size_t)nk2 * nk1, hipMemcpyHostToDevice, st));
         reset_history();
     }
 
     void reset_history()
     {
         for (int c = 0; c < 3; c++) {
-            HIPCHK(hipMemsetAsync(R[c], 0, sizeof(cx<double>) * nmodes, st));
-            if (sa) HIPCHK(hipMemsetAsync(ukprev[c], 0, sizeof(cx<double>) * nmodes, st));
+            HIPCHK(hipMemsetAsync(R[c], 0, sizeof(cx<double>) * nmodes_p, st));
+            if (sa) HIPCHK(hipMemsetAsync(ukprev[c], 0, sizeof(cx<double>) * nmodes_p, st));
         }
         DevScalars s = read_scalars();
         s.time = 0.0;
@@ -507,7 +523,8 @@ struct spns_ctx {
     {
         std::vector<std::complex<double>> h(3 * (size_t)nmodes);
         for (int c = 0; c < 3; c++)
-            HIPCHK(hipMemcpyAsync(h.data() + c * nmodes, src[c], sizeof(cx<double>) * nmodes, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpy2DAsync(h.data() + c * nmodes, sizeof(cx<double>) * nk3, src[c], sizeof(cx<double>) * nk3p,
+                                    sizeof(cx<double>) * nk3, (size_t)nk2 * nk1, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         return h;
     }
@@ -549,30 +566,13 @@ struct spns_ctx {
     {
         ShellArgs s{};
         for (int c = 0; c < 3; c++) s.uk[c] = uk[c];
-        s.nk1 = nk1; s.nk2 = nk2; s.nk3 = nk3; s.nmodes = nmodes;
+        s.nk1 = nk1; s.nk2 = nk2; s.nk3 = nk3; s.nk3p = nk3p; s.nmodes = nmodes_p;
         s.notcubic = cfg.isNotCubic; s.Lx = cfg.Lx; s.Ly = cfg.Ly; s.Lz = cfg.Lz;
         return s;
     }
 };
 
 namespace {
-
-// Counter-based generator for the synthetic initial field: splitmix64 of a
-// key built from (seed, k1, k2, k3, draw), so the field is independent of
-// any decomposition or batch.
-inline uint64_t splitmix(uint64_t x)
-{
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
-inline double unif(uint64_t seed, int k1, int k2, int k3, int d)
-{
-    uint64_t key = splitmix(seed) ^ splitmix(((uint64_t)(uint32_t)(k1 + 65536) << 40) ^ ((uint64_t)(uint32_t)(k2 + 65536) << 20) ^
-                                             (uint64_t)(uint32_t)k3 ^ ((uint64_t)d << 58));
-    return ((double)(splitmix(key) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-}
 
 int finish(spns_ctx* c, const std::exception& e, int code)
 {
@@ -776,47 +776,20 @@ int spns_init_random(spns_ctx* ctx, uint64_t seed, double k0, double urms)
 {
     API_BEGIN(ctx)
     if (!(k0 > 0) || !(urms >= 0)) throw Error(SPNS_EINVAL, "k0 must be > 0 and urms >= 0");
-    const int64_t nm = ctx->nmodes;
-    std::vector<std::complex<double>> h(3 * (size_t)nm, 0.0);
-    double etot = 0.0;
-    for (int k2 = -ctx->Ny_2; k2 <= ctx->Ny_2; k2++)
-        for (int k1 = -ctx->Nx_2; k1 <= ctx->Nx_2; k1++)
-            for (int k3 = 0; k3 <= ctx->Nz_2; k3++) {
-                if (!ctx->retained(k1, k2, k3)) continue;
-                const bool canon = k3 > 0 || k1 > 0 || (k1 == 0 && k2 > 0);
-                if (!canon) continue;
-                const double kk[3] = {(double)k1, (double)k2, (double)k3};
-                const double r2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
-                const double kmag = std::sqrt(r2);
-                std::complex<double> g[3];
-                for (int c = 0; c < 3; c++) {
-                    double u1 = unif(seed, k1, k2, k3, 2 * c), u2 = unif(seed, k1, k2, k3, 2 * c + 1);
-                    double rr = std::sqrt(-2.0 * std::log(u1));
-                    g[c] = std::complex<double>(rr * std::cos(2 * kPi * u2), rr * std::sin(2 * kPi * u2));
-                }
-                std::complex<double> kd = (kk[0] * g[0] + kk[1] * g[1] + kk[2] * g[2]) / r2;
-                double nrm = 0.0;
-                for (int c = 0; c < 3; c++) { g[c] -= kd * kk[c]; nrm += std::norm(g[c]); }
-                if (nrm <= 0) continue;
-                const double Ek = std::pow(kmag, 4.0) * std::exp(-2.0 * (kmag / k0) * (kmag / k0));
-                const double amp = std::sqrt(Ek / (4.0 * kPi * r2) / nrm);
-                const int64_t q = ctx->compact_index(k1, k2, k3);
-                for (int c = 0; c < 3; c++) h[c * nm + q] = g[c] * amp;
-                if (k3 == 0) {
-                    const int64_t qm = ctx->compact_index(-k1, -k2, 0);
-                    for (int c = 0; c < 3; c++) h[c * nm + qm] = std::conj(g[c] * amp);
-                }
-            }
-    for (int64_t q = 0; q < nm; q++) {
-        int i3 = (int)(q % ctx->nk3);
-        double w = (i3 != 0) ? 2.0 : 1.0;
-        double ke = 0.5 * (std::norm(h[q]) + std::norm(h[nm + q]) + std::norm(h[2 * nm + q]));
-        etot += w * ke;
-    }
-    const double target = 1.5 * urms * urms;
-    const double sc = etot > 0 ? std::sqrt(target / etot) : 0.0;
-    for (auto& z : h) z *= sc;
-    ctx->upload_state(h);
+    RandArgs r{};
+    for (int c = 0; c < 3; c++) r.uk[c] = ctx->uk[c];
+    r.nk1 = ctx->nk1; r.nk2 = ctx->nk2; r.nk3 = ctx->nk3; r.nk3p = ctx->nk3p; r.nmodes = ctx->nmodes_p;
+    r.notcubic = ctx->cfg.isNotCubic; r.Lx = ctx->cfg.Lx; r.Ly = ctx->cfg.Ly; r.Lz = ctx->cfg.Lz;
+    r.seed = seed;
+    r.k0 = k0;
+    HIPCHK(launch_init_random(r, ctx->st));
+    ctx->reduce_state();
+    DevScalars s = ctx->read_scalars();
+    const double target = 1.5 * urms * urms;  // u_rms^2 = <u_i u_i>/3
+    const double sc = s.ek > 0 ? std::sqrt(target / s.ek) : 0.0;
+    HIPCHK(launch_scale_state(ctx->shell_args(), sc, ctx->st));
+    ctx->reset_history();
+    ctx->reduce_state();
     API_END(ctx)
 }
 
@@ -958,10 +931,60 @@ int spns_velocity_physical(spns_ctx* ctx, double* uvw)
     API_END(ctx)
 }
 
+int spns_recalculate_velocity_antitransform(spns_ctx* ctx)
+{
+    API_BEGIN(ctx)
+    ctx->velocity_pass(nullptr);
+    API_END(ctx)
+}
+
 int spns_synchronize(spns_ctx* ctx)
 {
     API_BEGIN(ctx)
     HIPCHK(hipStreamSynchronize(ctx->st));
+    API_END(ctx)
+}
+
+int spns_profile_passes(spns_ctx* ctx, int32_t nsteps, double* ms)
+{
+    API_BEGIN(ctx)
+    if (nsteps < 1 || !ms) throw Error(SPNS_EINVAL, "nsteps >= 1 and an output array are required");
+    constexpr int NP = 8;
+    hipEvent_t ev[NP + 1];
+    for (int i = 0; i <= NP; i++) HIPCHK(hipEventCreate(&ev[i]));
+    double acc[NP] = {0};
+    try {
+        for (int s = 0; s < nsteps; s++) {
+            HIPCHK(hipEventRecord(ev[0], ctx->st));
+            HIPCHK(launch_clear(ctx->mv, ctx->mv + 1, ctx->st));
+            HIPCHK(hipEventRecord(ev[1], ctx->st));
+            ctx->inv_y(false);
+            HIPCHK(hipEventRecord(ev[2], ctx->st));
+            ctx->inv_x(false);
+            HIPCHK(hipEventRecord(ev[3], ctx->st));
+            ctx->zpass(0, nullptr);
+            HIPCHK(hipEventRecord(ev[4], ctx->st));
+            ctx->fwd_x();
+            HIPCHK(hipEventRecord(ev[5], ctx->st));
+            ctx->fwd_y();
+            HIPCHK(hipEventRecord(ev[6], ctx->st));
+            HIPCHK(launch_epilogue(ctx->prec, ctx->epi_args(EPI_STEP), ctx->grid_epi, ctx->st));
+            HIPCHK(hipEventRecord(ev[7], ctx->st));
+            HIPCHK(launch_finalize(ctx->fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE), ctx->st));
+            HIPCHK(hipEventRecord(ev[8], ctx->st));
+            HIPCHK(hipEventSynchronize(ev[8]));
+            for (int i = 0; i < NP; i++) {
+                float f = 0.f;
+                HIPCHK(hipEventElapsedTime(&f, ev[i], ev[i + 1]));
+                acc[i] += f;
+            }
+        }
+    } catch (...) {
+        for (int i = 0; i <= NP; i++) hipEventDestroy(ev[i]);
+        throw;
+    }
+    for (int i = 0; i <= NP; i++) hipEventDestroy(ev[i]);
+    for (int i = 0; i < NP; i++) ms[i] = acc[i] / nsteps;
     API_END(ctx)
 }
 

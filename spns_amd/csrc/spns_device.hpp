@@ -145,6 +145,41 @@ __device__ __forceinline__ void dft16(cx<T>* x)
     }
 }
 
+// 24 = 3 x 8: three 8-point DFTs of the decimated sequences, twiddles W24^(n2 k),
+// then 3-point DFTs across n2.
+template <int DIR, typename T>
+__device__ __forceinline__ void dft24(cx<T>* x)
+{
+    // cos/sin of k*15 degrees, k = 0..7
+    const T C[8] = {(T)1.0, (T)0.96592582628906828675, (T)0.86602540378443864676, (T)0.70710678118654752440,
+                    (T)0.5, (T)0.25881904510252076235, (T)0.0, (T)-0.25881904510252076235};
+    const T S[8] = {(T)0.0, (T)0.25881904510252076235, (T)0.5, (T)0.70710678118654752440,
+                    (T)0.86602540378443864676, (T)0.96592582628906828675, (T)1.0, (T)0.96592582628906828675};
+    cx<T> d[3][8];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; n2++) {
+#pragma unroll
+        for (int n1 = 0; n1 < 8; n1++) d[n2][n1] = x[3 * n1 + n2];
+        dft8<DIR>(d[n2]);
+    }
+    // W24^(2k): cos/sin of k*30 degrees, k = 0..7
+    const T C2[8] = {(T)1.0, (T)0.86602540378443864676, (T)0.5, (T)0.0, (T)-0.5, (T)-0.86602540378443864676, (T)-1.0, (T)-0.86602540378443864676};
+    const T S2[8] = {(T)0.0, (T)0.5, (T)0.86602540378443864676, (T)1.0, (T)0.86602540378443864676, (T)0.5, (T)0.0, (T)-0.5};
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        d[1][k] = cmul(d[1][k], wk<DIR>(C[k], S[k]));
+        d[2][k] = cmul(d[2][k], wk<DIR>(C2[k], S2[k]));
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        cx<T> a0 = d[0][k], a1 = d[1][k], a2 = d[2][k];
+        dft3<DIR>(a0, a1, a2);
+        x[k] = a0;
+        x[k + 8] = a1;
+        x[k + 16] = a2;
+    }
+}
+
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dft(cx<T>* x)
 {
@@ -155,6 +190,7 @@ __device__ __forceinline__ void dft(cx<T>* x)
     else if constexpr (R == 8) dft8<DIR>(x);
     else if constexpr (R == 12) dft12<DIR>(x);
     else if constexpr (R == 16) dft16<DIR>(x);
+    else if constexpr (R == 24) dft24<DIR>(x);
     else static_assert(R == 2, "unsupported radix");
 }
 
@@ -274,6 +310,118 @@ struct Fft {
     __device__ static __forceinline__ void run(C* __restrict__ lds, const C* __restrict__ tw, Load load, Store store)
     {
         __syncthreads();  // LDS may still be read by a previous use
+        stage<0>(lds, tw, load, store);
+    }
+};
+
+
+// ---------------------------------------------------------------------------
+// One-wave FFT of a single length-L column (L = 64*E).  The first radix is E,
+// so stage 0 reads points t + 64 r (r < E) of lane t: one coalesced load per
+// r.  Every later radix divides E, so the last stage leaves lane t holding
+// exactly the points t + 64 m (m < E); the store functor receives m as a
+// compile-time slot and can keep the values in registers.  Exchanges use the
+// wave's own LDS column with wave-level ordering only (no block barrier).
+// ---------------------------------------------------------------------------
+template <int L>
+struct WavePlan;
+#define SPNS_WPLAN(LEN, ...)                                               \
+    template <>                                                           \
+    struct WavePlan<LEN> {                                                \
+        static constexpr int n = sizeof((int[]){__VA_ARGS__}) / sizeof(int); \
+        __host__ __device__ static constexpr int r(int s)                 \
+        {                                                                 \
+            constexpr int a[] = {__VA_ARGS__};                            \
+            return a[s];                                                  \
+        }                                                                 \
+    };
+SPNS_WPLAN(512, 8, 8, 8)
+SPNS_WPLAN(768, 12, 4, 4, 4)
+SPNS_WPLAN(1024, 16, 8, 8)
+SPNS_WPLAN(1536, 24, 8, 8)
+#undef SPNS_WPLAN
+
+template <int L>
+struct has_wave_plan {
+    static constexpr bool value = (L == 512 || L == 768 || L == 1024 || L == 1536);
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, int L, int DIR>
+struct WaveFft {
+    using C = cx<T>;
+    static constexpr int E = L / 64;
+    static constexpr int NS = WavePlan<L>::n;
+
+    template <int S>
+    __host__ __device__ static constexpr int span()
+    {
+        int p = 1;
+        for (int s = 0; s < S; s++) p *= WavePlan<L>::r(s);
+        return p;
+    }
+
+    template <int S, class Load, class Store>
+    __device__ static __forceinline__ void stage(C* __restrict__ lds, const C* tw, Load& load, Store& store)
+    {
+        constexpr int R = WavePlan<L>::r(S);
+        constexpr int Ns = span<S>();
+        constexpr int LR = L / R;
+        constexpr int BPT = LR / 64;
+        constexpr int TWS = L / (Ns * R);
+        static_assert(LR % 64 == 0, "wave plan: every radix must divide L/64");
+        const int t = threadIdx.x & 63;
+        C v[BPT][R];
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int j = t + 64 * q;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if constexpr (S == 0) v[q][r] = load(j + r * LR, r);
+                else v[q][r] = lds[j + r * LR];
+            }
+        }
+        if constexpr (S > 0) wave_sync();
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int j = t + 64 * q;
+            const int k = j % Ns;
+            if constexpr (S > 0) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    C w = tw[r * k * TWS];
+                    if (DIR > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft<R, DIR>(v[q]);
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if constexpr (S == NS - 1) store(base + r * Ns, v[q][r], q + r * (E / R));
+                else lds[base + r * Ns] = v[q][r];
+            }
+        }
+        if constexpr (S < NS - 1) {
+            wave_sync();
+            stage<S + 1>(lds, tw, load, store);
+        }
+    }
+
+    template <class Load, class Store>
+    __device__ static __forceinline__ void run(C* __restrict__ lds, const C* tw, Load load, Store store)
+    {
+        // The twiddles depend only on the lane; without this compiler barrier
+        // they are hoisted out of the callers' loops and stay live in
+        // registers for the whole kernel.
+        asm volatile("" ::: "memory");
+        wave_sync();
         stage<0>(lds, tw, load, store);
     }
 };

@@ -40,9 +40,11 @@ struct StridedArgs {
 };
 
 struct ZArgs {
-    void* f[9];  // X2 fields: u v w | ux vx wx | uy vy wy ; pencil offset pen*nk3
-    int nk3;
+    void* f[9];  // X2 fields: u v w | ux vx wx | uy vy wy ; pencil offset pen*pitch
+    int nk3, pitch;
     int les;
+    int sa;  // also reduce the self-adaptive sum form of the CFL velocity
+    int kernel;  // 0: wave-per-pencil where a wave plan exists, 1: block kernel
     int64_t npencil;
     double Ax, Ay, Az;  // CFL divisors (src/HIT_LES.h:56-58)
     double ct;          // (C_Smag*delta)^2 (src/HIT_LES.cpp:132)
@@ -67,8 +69,8 @@ struct EpiArgs {
     cx<double>* uk[3];
     cx<double>* R[3];
     cx<double>* ukprev[3];
-    int nk1, nk2, nk3, nbins;
-    int64_t nmodes;
+    int nk1, nk2, nk3, nk3p, nbins;
+    int64_t nmodes;  // padded: nk2 * nk1 * nk3p
     int notcubic, Lx, Ly, Lz, Mz;
     int mode, les, rot, force_on, kfmin, kfmax, sa;
     double nf;  // Mx*My*Mz
@@ -99,8 +101,8 @@ struct FinArgs {
 
 struct ShellArgs {
     cx<double>* uk[3];
-    int nk1, nk2, nk3;
-    int64_t nmodes;
+    int nk1, nk2, nk3, nk3p;
+    int64_t nmodes;  // padded
     int notcubic, Lx, Ly, Lz;
     const double* ft;
     int last_eff, nullify;
@@ -113,7 +115,18 @@ struct NormArgs {
     double nf;
 };
 
+struct RandArgs {
+    cx<double>* uk[3];
+    int nk1, nk2, nk3, nk3p;
+    int64_t nmodes;  // padded
+    int notcubic, Lx, Ly, Lz;
+    uint64_t seed;
+    double k0;
+};
+
 bool supported_length(int L);
+hipError_t launch_init_random(const RandArgs& a, hipStream_t st);
+hipError_t launch_scale_state(const ShellArgs& a, double sc, hipStream_t st);
 int strided_tile(int L, int prec);
 int zpass_threads(int L);
 hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st);

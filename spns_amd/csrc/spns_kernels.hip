@@ -144,47 +144,59 @@ __global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
     const cx<T>* tw = (const cx<T>*)a.tw;
     const int nk = a.nk3;
     const int t = threadIdx.x;
-    double vmax = 0.0, vmax_sa = 0.0, s2 = 0.0, s3 = 0.0;
+    T mu = (T)0, mv = (T)0, mw = (T)0;
+    double vmax_sa = 0.0, s2 = 0.0, s3 = 0.0;
     const double iAx = a.Ax, iAy = a.Ay, iAz = a.Az;
+    const bool sa = a.sa;
+
+    // Source of each logical field f (0 u,1 v,2 w | 3 ux,4 uy,5 uz | 6 vx,7 vy,8 vz
+    // | 9 wx,10 wy,11 wz): X2 slot and whether it is d_z (i k3 times a
+    // velocity spectrum).  Resolved once per block so the loads below are
+    // branch-free and use no per-lane kernel-argument indexing.
+    __shared__ const cx<T>* srcp[12];
+    __shared__ int sderiv[12];
+    if (t < 12) {
+        const int f = t;
+        const cx<T>* p = nullptr;
+        int dz = 0;
+        if (f < 3) p = (const cx<T>*)a.f[f];
+        else {
+            const int c = (f - 3) / 3, d = (f - 3) % 3;
+            if (d == 0) p = (const cx<T>*)a.f[3 + c];
+            else if (d == 1) p = (const cx<T>*)a.f[6 + c];
+            else { p = (const cx<T>*)a.f[c]; dz = 1; }
+        }
+        if (MODE == 1 && f >= 3) p = nullptr;  // velocity mode: pairs (u,v), (w,0)
+        srcp[f] = p;
+        sderiv[f] = dz;
+    }
+    __syncthreads();
 
     for (int64_t pen = blockIdx.x; pen < a.npencil; pen += gridDim.x) {
-        const int64_t off = pen * nk;
-        // -------- inverse: build paired Hermitian spectra and transform
-        auto field = [&](int f, int k) -> cx<T> {
-            // logical field f at k3=k (0<=k<nk):
-            // 0 u,1 v,2 w | 3 ux,4 uy,5 uz | 6 vx,7 vy,8 vz | 9 wx,10 wy,11 wz
-            if (f < 3) {
-                cx<T> v = ((const cx<T>*)a.f[f])[off + k];
-                return v;
-            }
-            const int c = (f - 3) / 3, d = (f - 3) % 3;
-            if (d == 0) return ((const cx<T>*)a.f[3 + c])[off + k];
-            if (d == 1) return ((const cx<T>*)a.f[6 + c])[off + k];
-            cx<T> v = ((const cx<T>*)a.f[c])[off + k];
-            const T kk = (T)k;
-            return cx<T>{-kk * v.y, kk * v.x};
-        };
+        const int64_t off = pen * a.pitch;
+        // -------- inverse: build paired Hermitian spectra and transform.
+        // Z[p] = A[p] + i B[p] for 0 <= p < nk, conj(A[L-p]) + i conj(B[L-p])
+        // for L-p < nk; only Re of the p = 0 terms enters (as in a c2r).
         constexpr int NPI = (MODE == 0) ? 6 : 2;
         auto load_inv = [&](int b, int p) -> cx<T> {
+            const bool lo = p < nk, hi = p > L - nk;
+            if (!(lo || hi)) return cx<T>{(T)0, (T)0};
+            const int k = lo ? p : L - p;
+            const T kk = (T)k;
             const int fa = 2 * b, fb = 2 * b + 1;
-            const bool hasb = (MODE == 0) || b == 0;  // velocity mode: (u,v), (w,0)
-            if (p == 0) {
-                cx<T> A = field(fa, 0);
-                T bx = hasb ? field(fb, 0).x : (T)0;
-                return cx<T>{A.x, bx};
+            const cx<T>* pa = srcp[fa];
+            const cx<T>* pb = srcp[fb];
+            cx<T> A = pa[off + k];
+            if (sderiv[fa]) A = cx<T>{-kk * A.y, kk * A.x};
+            cx<T> B = cx<T>{(T)0, (T)0};
+            if (pb) {
+                B = pb[off + k];
+                if (sderiv[fb]) B = cx<T>{-kk * B.y, kk * B.x};
             }
-            if (p < nk) {
-                cx<T> A = field(fa, p);
-                cx<T> B = hasb ? field(fb, p) : cx<T>{(T)0, (T)0};
-                return cx<T>{A.x - B.y, A.y + B.x};
-            }
-            if (p > L - nk) {
-                const int k = L - p;
-                cx<T> A = field(fa, k);
-                cx<T> B = hasb ? field(fb, k) : cx<T>{(T)0, (T)0};
-                return cx<T>{A.x + B.y, B.x - A.y};
-            }
-            return cx<T>{(T)0, (T)0};
+            const T s = lo ? (T)1 : (T)-1;
+            const T ay = (p == 0) ? (T)0 : s * A.y;
+            const T by = (p == 0) ? (T)0 : s * B.y;
+            return cx<T>{A.x - by, ay + B.x};
         };
         auto store_lds = [&](int b, int p, cx<T> v) { lds[p * LD + b] = v; };
         Fft<T, L, NPI, NT, LD, +1>::run(lds, tw, load_inv, store_lds);
@@ -194,12 +206,10 @@ __global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
         for (int n = t; n < L; n += NT) {
             cx<T>* row = &lds[n * LD];
             const T u = row[0].x, v = row[0].y, w = row[1].x;
-            {
-                const double au = fabs((double)u) / iAx, av = fabs((double)v) / iAy, aw = fabs((double)w) / iAz;
-                const double m = fmax(au, fmax(av, aw));  // src/HIT_LES.cpp:393
-                vmax = fmax(vmax, m);
-                vmax_sa = fmax(vmax_sa, au + av + aw);  // src/HIT_LES.cpp:377
-            }
+            mu = fmax(mu, fabs(u));
+            mv = fmax(mv, fabs(v));
+            mw = fmax(mw, fabs(w));
+            if (sa) vmax_sa = fmax(vmax_sa, fabs((double)u) / iAx + fabs((double)v) / iAy + fabs((double)w) / iAz);  // src/HIT_LES.cpp:377
             if constexpr (MODE == 1) {
                 if (a.phys) {
                     // export physical velocity as [c][y][x][z]
@@ -270,13 +280,184 @@ __global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
         __syncthreads();
     }
     // -------- block reductions -> partials / atomics
-    vmax = block_max<NT>(vmax, red);
-    vmax_sa = block_max<NT>(vmax_sa, red);
+    const double bu = block_max<NT>((double)mu, red);
+    const double bv = block_max<NT>((double)mv, red);
+    const double bw = block_max<NT>((double)mw, red);
+    const double vmax = fmax(bu / iAx, fmax(bv / iAy, bw / iAz));  // src/HIT_LES.cpp:393
+    if (sa) vmax_sa = block_max<NT>(vmax_sa, red);
     if (MODE == 0) {
         s2 = block_sum<NT>(s2, red);
         s3 = block_sum<NT>(s3, red);
     }
     if (t == 0) {
+        atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
+        atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
+        if (MODE == 0) {
+            a.skew_partial[2 * blockIdx.x] = s2;
+            a.skew_partial[2 * blockIdx.x + 1] = s3;
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// z pass, one wave per pencil (DNS and velocity-only modes, L = 64*E).
+// Lane t owns the physical points z = t + 64 m; the twelve fields arrive pair
+// by pair from WaveFft and the products u.grad(u) accumulate in registers in
+// the reference's operand order ((u g1 + v g2) + w g3, src/HIT_LES.cpp:119).
+// The forward transforms read their first stage straight from those
+// registers; only the final unpack goes through the wave's LDS column.
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+__device__ __forceinline__ void wave_inv_pair(cx<T>* lds, const cx<T>* tw, const cx<T>* pa, int da, const cx<T>* pb, int db,
+                                              int64_t off, int nk, T* outA, T* outB)
+{
+    auto load = [&](int p, int slot) -> cx<T> {
+        (void)slot;
+        const bool lo = p < nk, hi = p > L - nk;
+        if (!(lo || hi)) return cx<T>{(T)0, (T)0};
+        const int k = lo ? p : L - p;
+        const T kk = (T)k;
+        cx<T> A = pa[off + k];
+        if (da) A = cx<T>{-kk * A.y, kk * A.x};
+        cx<T> B = cx<T>{(T)0, (T)0};
+        if (pb) {
+            B = pb[off + k];
+            if (db) B = cx<T>{-kk * B.y, kk * B.x};
+        }
+        const T sg = lo ? (T)1 : (T)-1;
+        const T ay = (p == 0) ? (T)0 : sg * A.y;
+        const T by = (p == 0) ? (T)0 : sg * B.y;
+        return cx<T>{A.x - by, ay + B.x};
+    };
+    auto store = [&](int p, cx<T> v, int slot) {
+        (void)p;
+        outA[slot] = v.x;
+        outB[slot] = v.y;
+    };
+    WaveFft<T, L, +1>::run(lds, tw, load, store);
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void wave_fwd_pair(cx<T>* lds, const cx<T>* tw, const T* inA, const T* inB, cx<T>* qa, cx<T>* qb,
+                                              int64_t off, int nk)
+{
+    auto load = [&](int p, int slot) -> cx<T> {
+        (void)p;
+        return cx<T>{inA[slot], inB ? inB[slot] : (T)0};
+    };
+    auto store = [&](int p, cx<T> v, int slot) {
+        (void)slot;
+        lds[p] = v;
+    };
+    WaveFft<T, L, -1>::run(lds, tw, load, store);
+    wave_sync();
+    const int t = threadIdx.x & 63;
+    for (int k = t; k < nk; k += 64) {
+        const cx<T> Z = lds[k], Zc = lds[(L - k) % L];
+        const T h = (T)0.5;
+        qa[off + k] = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
+        if (qb) qb[off + k] = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
+    }
+}
+
+template <typename T, int L, int MODE>
+__global__ __launch_bounds__(256) void k_zwave(ZArgs a)
+{
+    constexpr int E = L / 64, WPB = 4;
+    __shared__ cx<T> ldsw[WPB][L];
+    __shared__ double red[256];
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    cx<T>* lds = ldsw[w];
+    const cx<T>* tw = (const cx<T>*)a.tw;
+    const int nk = a.nk3;
+    const cx<T>* F[9];
+#pragma unroll
+    for (int f = 0; f < 9; f++) F[f] = (const cx<T>*)a.f[f];
+    // max over points of max(|u|/Ax, |v|/Ay, |w|/Az) equals the max of the
+    // per-component maxima divided once (correctly rounded division by a
+    // positive constant is monotonic); the sum form needs every point.
+    T mu = (T)0, mv = (T)0, mw = (T)0;
+    double vmax_sa = 0.0, s2 = 0.0, s3 = 0.0;
+    const double iAx = a.Ax, iAy = a.Ay, iAz = a.Az;
+    const bool sa = a.sa;
+
+    for (int64_t pen = (int64_t)blockIdx.x * WPB + w; pen < a.npencil; pen += (int64_t)gridDim.x * WPB) {
+        const int64_t off = pen * a.pitch;
+        T U[E], V[E], W[E];
+        wave_inv_pair<T, L>(lds, tw, F[0], 0, F[1], 0, off, nk, U, V);  // (u, v)
+        if constexpr (MODE == 1) {
+            T Z0[E];
+            wave_inv_pair<T, L>(lds, tw, F[2], 0, nullptr, 0, off, nk, W, Z0);  // (w, -)
+        }
+        if constexpr (MODE == 1) {
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                mu = fmax(mu, fabs(U[m]));
+                mv = fmax(mv, fabs(V[m]));
+                mw = fmax(mw, fabs(W[m]));
+            }
+            if (sa) {
+                for (int m = 0; m < E; m++)
+                    vmax_sa = fmax(vmax_sa, fabs((double)U[m]) / iAx + fabs((double)V[m]) / iAy + fabs((double)W[m]) / iAz);
+            }
+            if (a.phys) {
+                const int64_t P = a.npencil * L;
+                T* o = (T*)a.phys;
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    const int64_t q = pen * L + t + 64 * m;
+                    o[q] = U[m];
+                    o[P + q] = V[m];
+                    o[2 * P + q] = W[m];
+                }
+            }
+        } else {
+            T CX[E], CY[E], CZ[E], G1[E], G2[E];
+            wave_inv_pair<T, L>(lds, tw, F[2], 0, F[3], 0, off, nk, W, G1);  // (w, ux)
+            if (sa) {
+                for (int m = 0; m < E; m++)
+                    vmax_sa = fmax(vmax_sa, fabs((double)U[m]) / iAx + fabs((double)V[m]) / iAy + fabs((double)W[m]) / iAz);
+            }
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                mu = fmax(mu, fabs(U[m]));
+                mv = fmax(mv, fabs(V[m]));
+                mw = fmax(mw, fabs(W[m]));
+                const double g = (double)G1[m];
+                s2 += g * g;
+                s3 += g * g * g;
+                CX[m] = U[m] * G1[m];
+            }
+            wave_inv_pair<T, L>(lds, tw, F[6], 0, F[0], 1, off, nk, G1, G2);  // (uy, uz)
+#pragma unroll
+            for (int m = 0; m < E; m++) CX[m] = CX[m] + V[m] * G1[m] + W[m] * G2[m];
+            wave_inv_pair<T, L>(lds, tw, F[4], 0, F[7], 0, off, nk, G1, G2);  // (vx, vy)
+#pragma unroll
+            for (int m = 0; m < E; m++) CY[m] = U[m] * G1[m] + V[m] * G2[m];
+            wave_inv_pair<T, L>(lds, tw, F[1], 1, F[5], 0, off, nk, G1, G2);  // (vz, wx)
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                CY[m] = CY[m] + W[m] * G1[m];
+                CZ[m] = U[m] * G2[m];
+            }
+            wave_inv_pair<T, L>(lds, tw, F[8], 0, F[2], 1, off, nk, G1, G2);  // (wy, wz)
+#pragma unroll
+            for (int m = 0; m < E; m++) CZ[m] = CZ[m] + V[m] * G1[m] + W[m] * G2[m];
+            wave_fwd_pair<T, L>(lds, tw, CX, CY, (cx<T>*)a.f[0], (cx<T>*)a.f[1], off, nk);
+            wave_fwd_pair<T, L>(lds, tw, CZ, (const T*)nullptr, (cx<T>*)a.f[2], (cx<T>*)nullptr, off, nk);
+        }
+    }
+    const double bu = block_max<256>((double)mu, red);
+    const double bv = block_max<256>((double)mv, red);
+    const double bw = block_max<256>((double)mw, red);
+    const double vmax = fmax(bu / iAx, fmax(bv / iAy, bw / iAz));  // src/HIT_LES.cpp:393
+    if (sa) vmax_sa = block_max<256>(vmax_sa, red);
+    if (MODE == 0) {
+        s2 = block_sum<256>(s2, red);
+        s3 = block_sum<256>(s3, red);
+    }
+    if (threadIdx.x == 0) {
         atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
         atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
         if (MODE == 0) {
@@ -375,10 +556,11 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
     const double k05 = 1.0 / (kap + 0.5);
     const bool first = a.scal->iter == 0;
     double om = 0.0;
-    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3;
+    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3, nk3p = a.nk3p;
     for (int64_t m = (int64_t)blockIdx.x * NT + threadIdx.x; m < a.nmodes; m += (int64_t)gridDim.x * NT) {
-        const int i3 = (int)(m % nk3);
-        const int64_t r12 = m / nk3;
+        const int i3 = (int)(m % nk3p);
+        if (i3 >= nk3) continue;  // pitch padding
+        const int64_t r12 = m / nk3p;
         const int i1 = (int)(r12 % nk1), i2 = (int)(r12 / nk1);
         const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2), k3 = i3;
         if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) continue;  // src/HIT_LES.cpp:867-871
@@ -504,10 +686,29 @@ __global__ __launch_bounds__(NT) void k_finalize(FinArgs a)
     DevScalars* s = a.scal;
     const int t = threadIdx.x;
     if (a.flags & FIN_REDUCE) {
-        for (int K = t; K < a.nbins; K += NT) {
-            double v = 0.0;
-            for (int b = 0; b < a.nblk_epi; b++) v += a.hist_partial[(int64_t)b * a.nbins + K];
-            a.ek[K] = v;
+        // E(K): bins are split over NT/QB thread groups; each thread sums a
+        // fixed quarter of the block partials with 4 independent
+        // accumulators, and the quarters are combined in a fixed order.
+        constexpr int QB = 4;
+        __shared__ double q[QB][NT / QB];
+        const int lane = t % (NT / QB), quarter = t / (NT / QB);
+        const int per = (a.nblk_epi + QB - 1) / QB;
+        const int b0 = quarter * per, b1 = min(a.nblk_epi, b0 + per);
+        for (int K0 = 0; K0 < a.nbins; K0 += NT / QB) {
+            const int K = K0 + lane;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            if (K < a.nbins) {
+                int b = b0;
+                for (; b + 3 < b1; b += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) acc[u] += a.hist_partial[(int64_t)(b + u) * a.nbins + K];
+                }
+                for (; b < b1; b++) acc[0] += a.hist_partial[(int64_t)b * a.nbins + K];
+            }
+            q[quarter][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            __syncthreads();
+            if (quarter == 0 && K < a.nbins) a.ek[K] = ((q[0][lane] + q[1][lane]) + q[2][lane]) + q[3][lane];
+            __syncthreads();
         }
         double om = 0.0;
         for (int b = t; b < a.nblk_epi; b += NT) om += a.om_partial[b];
@@ -526,11 +727,16 @@ __global__ __launch_bounds__(NT) void k_finalize(FinArgs a)
         }
     }
     if (a.flags & FIN_SKEW) {
-        double s2 = 0.0, s3 = 0.0;
-        for (int b = t; b < a.nblk_z; b += NT) { s2 += a.skew_partial[2 * b]; s3 += a.skew_partial[2 * b + 1]; }
-        s2 = block_sum<NT>(s2, red);
-        s3 = block_sum<NT>(s3, red);
-        if (t == 0) { s->skew2 = s2; s->skew3 = s3; }
+        double s2[4] = {0.0, 0.0, 0.0, 0.0}, s3[4] = {0.0, 0.0, 0.0, 0.0};
+        int b = t;
+        for (; b + 3 * NT < a.nblk_z; b += 4 * NT) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) { s2[u] += a.skew_partial[2 * (b + u * NT)]; s3[u] += a.skew_partial[2 * (b + u * NT) + 1]; }
+        }
+        for (; b < a.nblk_z; b += NT) { s2[0] += a.skew_partial[2 * b]; s3[0] += a.skew_partial[2 * b + 1]; }
+        double S2 = block_sum<NT>((s2[0] + s2[1]) + (s2[2] + s2[3]), red);
+        double S3 = block_sum<NT>((s3[0] + s3[1]) + (s3[2] + s3[3]), red);
+        if (t == 0) { s->skew2 = S2; s->skew3 = S3; }
     }
     if (t == 0) {
         if (a.flags & FIN_MAXVEL) {
@@ -583,10 +789,11 @@ static __global__ void k_forcing_ft(const double* ek, const double* forced, doub
 
 static __global__ void k_scale_shells(ShellArgs a)
 {
-    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3;
+    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3, nk3p = a.nk3p;
     for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < a.nmodes; m += (int64_t)gridDim.x * blockDim.x) {
-        const int i3 = (int)(m % nk3);
-        const int64_t r12 = m / nk3;
+        const int i3 = (int)(m % nk3p);
+        if (i3 >= nk3) continue;  // pitch padding
+        const int64_t r12 = m / nk3p;
         const int i1 = (int)(r12 % nk1), i2 = (int)(r12 / nk1);
         const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2), k3 = i3;
         if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) continue;
@@ -603,7 +810,7 @@ static __global__ void k_scale_shells(ShellArgs a)
 // Complex conjugate correction of the k3 = 0 plane (src/HIT_LES.cpp:446-476)
 static __global__ void k_ccc(ShellArgs a)
 {
-    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3;
+    const int nk1 = a.nk1, nk2 = a.nk2;
     const int n = nk1 * nk2;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const int i1 = q % nk1, i2 = q / nk1;
@@ -611,8 +818,8 @@ static __global__ void k_ccc(ShellArgs a)
         if (k1 < 0 || (k1 == 0 && k2 < 0)) {
             const int j1 = (-k1 >= 0) ? -k1 : nk1 - k1;
             const int j2 = (-k2 >= 0) ? -k2 : nk2 - k2;
-            const int64_t m = ((int64_t)i2 * nk1 + i1) * nk3;
-            const int64_t mc = ((int64_t)j2 * nk1 + j1) * nk3;
+            const int64_t m = ((int64_t)i2 * nk1 + i1) * a.nk3p;
+            const int64_t mc = ((int64_t)j2 * nk1 + j1) * a.nk3p;
             for (int c = 0; c < 3; c++) {
                 cx<double> v = a.uk[c][mc];
                 a.uk[c][m] = cx<double>{v.x, -v.y};
@@ -737,6 +944,12 @@ hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, c
 template <typename T, int L, int MODE>
 static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
 {
+    if constexpr (has_wave_plan<L>::value) {
+        if ((!a.les || MODE == 1) && a.kernel == 0) {
+            hipLaunchKernelGGL((k_zwave<T, L, MODE>), dim3(grid), dim3(256), 0, st, a);
+            return hipGetLastError();
+        }
+    }
     constexpr int NT = zpass_nt<L>();
     hipLaunchKernelGGL((k_zpass<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
     return hipGetLastError();
@@ -800,7 +1013,7 @@ hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st)
 
 hipError_t launch_finalize(const FinArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_finalize<256>), dim3(1), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_finalize<1024>), dim3(1), dim3(1024), 0, st, a);
     return hipGetLastError();
 }
 
@@ -827,6 +1040,99 @@ hipError_t launch_ccc(const ShellArgs& a, hipStream_t st)
     int n = a.nk1 * a.nk2;
     int grid = (n + 255) / 256;
     hipLaunchKernelGGL(k_ccc, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// Synthetic initial field: random-phase, solenoidal, E(k) ~ k^4 exp(-2(k/k0)^2)
+// (unscaled).  Counter-based: each mode's value depends only on
+// (seed, k1, k2, k3), so the field is the same for any launch shape.  The
+// k3 = 0 plane is made Hermitian by taking the mirror of the canonical half.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double unif(uint64_t seed, int k1, int k2, int k3, int d)
+{
+    uint64_t key = splitmix(seed) ^ splitmix(((uint64_t)(uint32_t)(k1 + 65536) << 40) ^
+                                             ((uint64_t)(uint32_t)(k2 + 65536) << 20) ^ (uint64_t)(uint32_t)k3 ^
+                                             ((uint64_t)d << 58));
+    return ((double)(splitmix(key) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+static __global__ void k_init_random(RandArgs a)
+{
+    const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3, nk3p = a.nk3p;
+    const double PI = 3.141592653589793238462643;
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < a.nmodes; m += (int64_t)gridDim.x * blockDim.x) {
+        const int i3 = (int)(m % nk3p);
+        if (i3 >= nk3) continue;  // pitch padding
+        const int64_t r12 = m / nk3p;
+        const int i1 = (int)(r12 % nk1), i2 = (int)(r12 / nk1);
+        int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2);
+        const int k3 = i3;
+        bool masked = a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz));
+        const bool canon = k3 > 0 || k1 > 0 || (k1 == 0 && k2 > 0);
+        const bool mirror = (k3 == 0) && !canon && !(k1 == 0 && k2 == 0);
+        cx<double> out[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+        if (!masked && (canon || mirror)) {
+            if (mirror) { k1 = -k1; k2 = -k2; }
+            const double K1 = k1, K2 = k2, K3 = k3;
+            const double r2 = K1 * K1 + K2 * K2 + K3 * K3;
+            double gr[3], gi[3];
+            for (int c = 0; c < 3; c++) {
+                const double u1 = unif(a.seed, k1, k2, k3, 2 * c), u2 = unif(a.seed, k1, k2, k3, 2 * c + 1);
+                const double rr = sqrt(-2.0 * log(u1));
+                gr[c] = rr * cos(2 * PI * u2);
+                gi[c] = rr * sin(2 * PI * u2);
+            }
+            const double dr = (K1 * gr[0] + K2 * gr[1] + K3 * gr[2]) / r2;
+            const double di = (K1 * gi[0] + K2 * gi[1] + K3 * gi[2]) / r2;
+            const double kk[3] = {K1, K2, K3};
+            double nrm = 0.0;
+            for (int c = 0; c < 3; c++) {
+                gr[c] -= dr * kk[c];
+                gi[c] -= di * kk[c];
+                nrm += gr[c] * gr[c] + gi[c] * gi[c];
+            }
+            if (nrm > 0.0) {
+                const double km = sqrt(r2);
+                const double Ek = km * km * km * km * exp(-2.0 * (km / a.k0) * (km / a.k0));
+                const double amp = sqrt(Ek / (4.0 * PI * r2) / nrm);
+                for (int c = 0; c < 3; c++) out[c] = cx<double>{gr[c] * amp, (mirror ? -1.0 : 1.0) * gi[c] * amp};
+            }
+        }
+        for (int c = 0; c < 3; c++) a.uk[c][m] = out[c];
+    }
+}
+
+static __global__ void k_scale_state(ShellArgs a, double sc)
+{
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < a.nmodes; m += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; c++) {
+            cx<double> v = a.uk[c][m];
+            a.uk[c][m] = cx<double>{v.x * sc, v.y * sc};
+        }
+}
+
+hipError_t launch_init_random(const RandArgs& a, hipStream_t st)
+{
+    int grid = (int)((a.nmodes + 255) / 256);
+    if (grid > 16384) grid = 16384;
+    hipLaunchKernelGGL(k_init_random, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_state(const ShellArgs& a, double sc, hipStream_t st)
+{
+    int grid = (int)((a.nmodes + 255) / 256);
+    if (grid > 16384) grid = 16384;
+    hipLaunchKernelGGL(k_scale_state, dim3(grid), dim3(256), 0, st, a, sc);
     return hipGetLastError();
 }
 

@@ -1,0 +1,75 @@
+"""spns_run, the MAIN.cpp-equivalent driver: parameter errors on the CPU
+(src/MAIN.cpp:98, 174-175, src/parser.cpp:364-479) and, on the GPU, the
+reference's QA logs reproduced from its own parameters files."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+import qa_driver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "spns_amd", "spns_run")
+ROW = re.compile(r"Time: ([\d.]+),\s+iter:\s+(\d+),\s+Dt: (\S+),\s+Ek: (\S+),\s+Om: (\S+),\s+ReLambda:\s+(\S+)")
+
+needs_run = pytest.mark.skipif(not os.path.exists(RUN), reason="spns_run not built")
+
+
+def write_params(d, params, extra=None):
+    lines = ["%s\t%s" % (k, v) for k, v in params.items()]
+    for k, v in (extra or {}).items():
+        lines.append("%s\t%s" % (k, v))
+    path = os.path.join(d, "params")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+@needs_run
+@pytest.mark.parametrize("text,msg", [
+    ("Ny 32\nNz 32\nFinal_Time 1\nnu 0.01\n", "Nx is missing"),
+    ("Nx 32\nNy 32\nNz 32\nFinal_Time 1\n", "Either kinematic viscosity or Reynolds lambda"),
+    ("Nx 32\nNy 32\nNz 32\nFinal_Time 1\nnu 0.01\nReynolds_lambda 100\n", "cannot be defined simoultaniously"),
+    ("Nx 32\nnx 16\nNy 32\nNz 32\nFinal_Time 1\nnu 0.01\n", "double definition"),
+])
+def test_parameter_errors(tmp_path, text, msg):
+    p = tmp_path / "params"
+    p.write_text(text)
+    r = subprocess.run([RUN, str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert msg in r.stderr
+
+
+@needs_run
+def test_missing_parameters_file(tmp_path):
+    r = subprocess.run([RUN, str(tmp_path / "nope")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Parameters file is missing" in r.stderr
+
+
+FIX = qa_driver.load_fixtures()
+
+
+@pytest.mark.gpu
+@needs_run
+@pytest.mark.parametrize("name", sorted(FIX))
+def test_driver_reproduces_qa_log(tmp_path, name):
+    fx = FIX[name]
+    params = dict(fx["params"])
+    os.makedirs(tmp_path / "DATA", exist_ok=True)
+    for src, dst in qa_driver.DATA_FILES.items():
+        shutil.copy(os.path.join(qa_driver.GOLDEN, dst), tmp_path / src)
+    if "InputEkFilename" in params:
+        shutil.copy(os.path.join(qa_driver.GOLDEN, "Forced_Energy_Cascade_K%s.dat" % params["Last_K"]),
+                    tmp_path / "DATA" / "Forced_Energy_Cascade.dat")
+    extra = {"K41_Ranks": 4}  # the REF logs were made with 4 MPI ranks
+    for k in ["Velocity_Physical", "Velocity_Fourier", "Energy_Cascade"]:
+        params[k] = "0"
+    path = write_params(str(tmp_path), params, extra)
+    r = subprocess.run([RUN, path, "--qa"], capture_output=True, text=True, cwd=str(tmp_path), timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = [list(m.groups()) for m in map(ROW.match, r.stdout.splitlines()) if m]
+    assert rows == fx["rows"]
+    finals = dict(l.split(":", 1) for l in r.stdout.splitlines() if l.startswith("Final "))
+    assert {k.strip(): v.strip() for k, v in finals.items()} == fx["final"]
