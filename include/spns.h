@@ -58,7 +58,8 @@ typedef struct spns_config {
     int32_t force_kmin, force_kmax;
     int32_t device;          /* HIP device ordinal */
     int32_t use_graph;       /* capture the step as a hipGraph */
-    int32_t reserved[6];
+    int32_t zpass_variant;   /* z-pass kernel: 0 default, 1 block per pencil, 2 wave per pencil */
+    int32_t reserved[5];
 } spns_config;
 
 typedef struct spns_scalars {
@@ -78,6 +79,9 @@ typedef struct spns_scalars {
 int spns_create(const spns_config *cfg, spns_ctx **out);
 /* Replaces HIT::~HIT / HIT_destroy (src/HIT_LES.cpp:1094-1130). */
 int spns_destroy(spns_ctx *ctx);
+/* Device memory a context with this configuration allocates (bytes), for
+ * admission by HBM footprint; -1 if the configuration is invalid. */
+int64_t spns_device_bytes(const spns_config *cfg);
 /* Number of compact modes per component, and of shells K=0..last_rad_max. */
 int64_t spns_nmodes(const spns_ctx *ctx);
 int32_t spns_nbins(const spns_ctx *ctx);

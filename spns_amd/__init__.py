@@ -36,7 +36,8 @@ class Config(ctypes.Structure):
         ("force_on", ctypes.c_int32), ("force_eps", ctypes.c_double),
         ("force_kmin", ctypes.c_int32), ("force_kmax", ctypes.c_int32),
         ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("zpass_variant", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -48,7 +49,7 @@ class Scalars(ctypes.Structure):
 
 
 EXPORTS = [
-    "spns_create", "spns_destroy", "spns_nmodes", "spns_nbins", "spns_last_rad", "spns_last_error",
+    "spns_create", "spns_destroy", "spns_device_bytes", "spns_nmodes", "spns_nbins", "spns_last_rad", "spns_last_error",
     "spns_set_state", "spns_get_state", "spns_get_rhist", "spns_init_from_physical", "spns_init_k41",
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
@@ -71,6 +72,7 @@ def load_library(path=LIB_PATH):
     sig = {
         "spns_create": (I32, [ctypes.POINTER(Config), ctypes.POINTER(P)]),
         "spns_destroy": (I32, [P]),
+        "spns_device_bytes": (I64, [ctypes.POINTER(Config)]),
         "spns_nmodes": (I64, [P]),
         "spns_nbins": (I32, [P]),
         "spns_last_rad": (I32, [P]),
@@ -123,6 +125,36 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def make_config(Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag=0.0, C_At=1.0, Lx=1, Ly=1, Lz=1,
+                omega=(0.0, 0.0, 0.0), isNotCubic=False, isSelfAdaptive=False, isCCC=False, precision=PREC_FP64,
+                delta=0.0, fixed_dt=0.0, force_eps=None, force_kmin=1, force_kmax=2, device=0, use_graph=True,
+                zpass_variant=0):
+    """spns_config for the arguments of Solver."""
+    Ny = Nx if Ny is None else Ny
+    Nz = Nx if Nz is None else Nz
+    c = Config()
+    c.Nx, c.Ny, c.Nz = Nx, Ny, Nz
+    c.Mx, c.My, c.Mz = Mx or 0, My or 0, Mz or 0
+    c.nu, c.C_Smag, c.C_At = nu, C_Smag, C_At
+    c.Lx, c.Ly, c.Lz = Lx, Ly, Lz
+    c.omega_x, c.omega_y, c.omega_z = omega
+    c.isNotCubic, c.isSelfAdaptive, c.isComplexConjugateCorrected = int(isNotCubic), int(isSelfAdaptive), int(isCCC)
+    c.precision = precision
+    c.delta, c.fixed_dt = delta, fixed_dt
+    c.force_on = int(force_eps is not None)
+    c.force_eps = 0.0 if force_eps is None else force_eps
+    c.force_kmin, c.force_kmax = force_kmin, force_kmax
+    c.device = device
+    c.use_graph = int(use_graph)
+    c.zpass_variant = zpass_variant
+    return c
+
+
+def device_bytes(**kw):
+    """Device memory a Solver with these arguments allocates."""
+    return int(load_library().spns_device_bytes(ctypes.byref(make_config(**kw))))
+
+
 class Solver:
     """One HIT run on the GPU.  Keyword arguments follow HIT::HIT
     (src/HIT_LES.h:23-29) after MAIN's length-factor expansion, plus
@@ -131,24 +163,13 @@ class Solver:
     def __init__(self, Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag=0.0, C_At=1.0,
                  Lx=1, Ly=1, Lz=1, omega=(0.0, 0.0, 0.0), isNotCubic=False, isSelfAdaptive=False,
                  isCCC=False, precision=PREC_FP64, delta=0.0, fixed_dt=0.0, force_eps=None,
-                 force_kmin=1, force_kmax=2, device=0, use_graph=True):
+                 force_kmin=1, force_kmax=2, device=0, use_graph=True, zpass_variant=0):
         self.L = load_library()
+        c = make_config(Nx, Ny, Nz, Mx, My, Mz, nu, C_Smag, C_At, Lx, Ly, Lz, omega, isNotCubic, isSelfAdaptive,
+                        isCCC, precision, delta, fixed_dt, force_eps, force_kmin, force_kmax, device, use_graph,
+                        zpass_variant)
         Ny = Nx if Ny is None else Ny
         Nz = Nx if Nz is None else Nz
-        c = Config()
-        c.Nx, c.Ny, c.Nz = Nx, Ny, Nz
-        c.Mx, c.My, c.Mz = Mx or 0, My or 0, Mz or 0
-        c.nu, c.C_Smag, c.C_At = nu, C_Smag, C_At
-        c.Lx, c.Ly, c.Lz = Lx, Ly, Lz
-        c.omega_x, c.omega_y, c.omega_z = omega
-        c.isNotCubic, c.isSelfAdaptive, c.isComplexConjugateCorrected = int(isNotCubic), int(isSelfAdaptive), int(isCCC)
-        c.precision = precision
-        c.delta, c.fixed_dt = delta, fixed_dt
-        c.force_on = int(force_eps is not None)
-        c.force_eps = 0.0 if force_eps is None else force_eps
-        c.force_kmin, c.force_kmax = force_kmin, force_kmax
-        c.device = device
-        c.use_graph = int(use_graph)
         self.cfg = c
         h = ctypes.c_void_p()
         rc = self.L.spns_create(ctypes.byref(c), ctypes.byref(h))

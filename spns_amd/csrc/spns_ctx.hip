@@ -216,7 +216,8 @@ struct spns_ctx {
         sa = cfg.isSelfAdaptive != 0;
         nfwd = les ? 9 : 3;
 
-        if (const char* zk = std::getenv("SPNS_ZPASS")) zkernel = std::string(zk) == "block" ? 1 : 0;
+        zkernel = cfg.zpass_variant == 2 ? 0 : 1;  // default: block kernel
+        if (const char* zk = std::getenv("SPNS_ZPASS")) zkernel = std::string(zk) == "wave" ? 0 : 1;
         HIPCHK(hipSetDevice(cfg.device));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ev0));
@@ -625,6 +626,20 @@ int spns_destroy(spns_ctx* ctx)
     ctx->release();
     delete ctx;
     return SPNS_OK;
+}
+
+int64_t spns_device_bytes(const spns_config* c)
+{
+    if (!c || c->Nx < 2 || c->Ny < 2 || c->Nz < 2) return -1;
+    const int64_t Mx = c->Mx > 0 ? c->Mx : (3 * c->Nx) / 2, My = c->My > 0 ? c->My : (3 * c->Ny) / 2;
+    const int64_t nk1 = 2 * (c->Nx / 2) + 1, nk2 = 2 * (c->Ny / 2) + 1, nk3 = c->Nz / 2 + 1;
+    const int64_t nk3p = (nk3 + 15) / 16 * 16;
+    const int64_t tsz = c->precision == SPNS_PREC_FP32_SPLIT ? 8 : 16;
+    const int64_t nfwd = c->C_Smag > 0 ? 9 : 3;
+    const int64_t state = 16 * nk2 * nk1 * nk3p * (c->isSelfAdaptive ? 9 : 6);
+    const int64_t x1 = std::max<int64_t>(6, nfwd) * My * nk1 * nk3p * tsz;
+    const int64_t x2 = 9 * My * Mx * nk3p * tsz;
+    return state + x1 + x2 + (64 << 20);  // + partial sums, twiddles, scalars
 }
 
 int64_t spns_nmodes(const spns_ctx* ctx) { return ctx ? ctx->nmodes : -1; }

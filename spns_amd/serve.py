@@ -1,0 +1,203 @@
+"""Serving loop: continuous batching of independent HIT runs on one GPU.
+
+Requests (independent runs of mixed grids, viscosities, Smagorinsky
+constants and rotation rates) arrive over time.  The loop
+  * admits a request when it has arrived and its device footprint
+    (spns_device_bytes) fits the HBM budget left by the runs in flight;
+  * keeps every admitted run resident: state, history and work buffers stay
+    in HBM from admission to completion;
+  * advances all runs together, round by round: each round enqueues a chunk
+    of hipGraph-captured steps for every active run on that run's own stream
+    before waiting, so the runs' kernels overlap on the device;
+  * retires runs that reached their step count (collecting E(k), energy,
+    dissipation and skewness), frees their memory and admits queued runs
+    into the freed space while the others continue.
+Latency = completion time - arrival time; the loop reports p50/p99 beside
+throughput.  The z-pass kernel variant of each shape class is picked once by
+a small timing autotuner (Tuner).
+
+Runs never share state: a run's result is the same alone and in the mix
+(tests/test_gpu_serve.py checks it bit for bit).
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import PREC_FP32_SPLIT, Solver, device_bytes
+
+GRIDS = [(32, 32, 32), (64, 64, 64), (96, 64, 64), (128, 128, 128), (192, 128, 128)]
+
+
+@dataclass
+class Request:
+    rid: int
+    seed: int
+    grid: tuple
+    C_Smag: float
+    nu: float
+    omega_z: float
+    steps: int
+    arrival: float
+    k0: float = 4.0
+    # filled by the server
+    admitted: float = math.nan
+    finished: float = math.nan
+    result: dict = field(default_factory=dict)
+
+    def solver_args(self, precision):
+        Nx, Ny, Nz = self.grid
+        return dict(Nx=Nx, Ny=Ny, Nz=Nz, nu=self.nu, C_Smag=self.C_Smag, C_At=0.5,
+                    omega=(0.0, 0.0, self.omega_z), precision=precision)
+
+
+def draw_requests(n, rate=20.0, seed=0, steps=(100, 2000)):
+    """The ensemble mix: Poisson arrivals at `rate`/s; each request drawn from
+    its own seed: grid uniform over GRIDS, DNS or Smagorinsky (C_s uniform in
+    [0.1, 0.2]), nu log-uniform in [1e-4, 1e-2], Omega_z in {0, 1, 5},
+    steps uniform in [steps[0], steps[1]]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        r = np.random.default_rng([seed, i])
+        grid = GRIDS[int(r.integers(len(GRIDS)))]
+        les = bool(r.integers(2))
+        out.append(Request(rid=i, seed=int(r.integers(1 << 31)), grid=grid,
+                           C_Smag=float(r.uniform(0.1, 0.2)) if les else 0.0,
+                           nu=float(10 ** r.uniform(-4, -2)), omega_z=float([0.0, 1.0, 5.0][int(r.integers(3))]),
+                           steps=int(r.integers(steps[0], steps[1] + 1)), arrival=t))
+    return out
+
+
+class Tuner:
+    """Picks the z-pass kernel variant (block or wave per pencil) of a shape
+    class by timing a few steps of each on a throwaway run."""
+
+    def __init__(self, steps=4):
+        self.steps = steps
+        self.choice = {}
+
+    def variant(self, args):
+        key = (args["Nx"], args["Ny"], args["Nz"], args.get("precision", 0), args.get("C_Smag", 0.0) > 0)
+        if key in self.choice:
+            return self.choice[key]
+        best, best_ms = 1, math.inf
+        for v in (1, 2):
+            s = Solver(zpass_variant=v, **args)
+            s.init_random(seed=0)
+            s.finish_init(False)
+            s.step(1)
+            s.synchronize()
+            s.step(self.steps)
+            ms = s.last_step_ms()
+            s.close()
+            if ms < best_ms:
+                best, best_ms = v, ms
+        self.choice[key] = best
+        return best
+
+
+class Server:
+    def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=64, tuner=None):
+        if hbm_budget is None:
+            hbm_budget = self.free_device_memory() * 0.8
+        self.budget = hbm_budget
+        self.precision = precision
+        self.chunk = chunk
+        self.max_active = max_active
+        self.tuner = tuner
+
+    @staticmethod
+    def free_device_memory():
+        try:
+            import torch
+            free, _ = torch.cuda.mem_get_info(0)
+            return float(free)
+        except Exception:
+            return 200e9
+
+    def _start(self, req, now):
+        args = req.solver_args(self.precision)
+        v = self.tuner.variant(args) if self.tuner else 0
+        s = Solver(zpass_variant=v, **args)
+        s.init_random(seed=req.seed, k0=req.k0)
+        s.finish_init(False)
+        req.admitted = now
+        return {"req": req, "solver": s, "done": 0, "bytes": device_bytes(zpass_variant=v, **args)}
+
+    def _finish(self, a, now):
+        s, req = a["solver"], a["req"]
+        sc = s.scalars()
+        req.result = {"ek": s.spectrum(), "energy": sc["ek"], "epsilon": sc["nu"] * sc["om"],
+                      "skewness": sc["skewness"], "time": sc["time"], "steps": sc["iter"], "state": s.get_state()}
+        req.finished = now
+        s.close()
+
+    def run(self, requests, keep_state=False):
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active, done = [], []
+        used = 0.0
+        t0 = time.perf_counter()
+        steps_done = 0
+        while pending or active:
+            now = time.perf_counter() - t0
+            # admission by HBM footprint, in arrival order
+            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                r = pending[0]
+                need = device_bytes(**r.solver_args(self.precision))
+                if used + need > self.budget and active:
+                    break
+                pending.pop(0)
+                a = self._start(r, now)
+                used += a["bytes"]
+                active.append(a)
+            if not active:
+                time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                continue
+            # one round: a chunk of captured steps for every active run, each on its own stream
+            for a in active:
+                n = min(self.chunk, a["req"].steps - a["done"])
+                a["solver"].step(n)
+                a["done"] += n
+                steps_done += n
+            for a in active:
+                a["solver"].synchronize()
+            now = time.perf_counter() - t0
+            still = []
+            for a in active:
+                if a["done"] >= a["req"].steps:
+                    self._finish(a, now)
+                    if not keep_state:
+                        a["req"].result.pop("state", None)
+                    used -= a["bytes"]
+                    done.append(a["req"])
+                else:
+                    still.append(a)
+            active = still
+        wall = time.perf_counter() - t0
+        lat = np.array([r.finished - r.arrival for r in done])
+        return {
+            "requests": len(done),
+            "wall_s": wall,
+            "runs_per_s": len(done) / wall,
+            "steps_per_s": steps_done / wall,
+            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else math.nan,
+            "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else math.nan,
+            "latency_mean_s": float(lat.mean()) if len(lat) else math.nan,
+        }, done
+
+
+def run_alone(req, precision=PREC_FP32_SPLIT, zpass_variant=0):
+    """Runs one request by itself (reference for the alone-vs-mix check)."""
+    s = Solver(zpass_variant=zpass_variant, **req.solver_args(precision))
+    s.init_random(seed=req.seed, k0=req.k0)
+    s.finish_init(False)
+    s.step(req.steps)
+    st = s.get_state()
+    sc = s.scalars()
+    ek = s.spectrum()
+    s.close()
+    return {"state": st, "ek": ek, "energy": sc["ek"], "time": sc["time"]}
