@@ -51,8 +51,16 @@ template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DI
 __global__ __launch_bounds__(NT) void k_strided(StridedArgs a)
 {
     __shared__ cx<T> lds[L * TB];
-    const int b0 = blockIdx.x * TB;
-    const int64_t ai = blockIdx.y;
+    // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+    // L2.  Remap the (padded, multiple-of-8) 1-D grid so every XCD owns a
+    // contiguous run of tiles: neighbouring column tiles, which share cache
+    // lines, then meet in the same L2.
+    const int ntiles = (a.B + TB - 1) / TB;
+    const int per = gridDim.x / 8;
+    const int lin = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int64_t ai = lin / ntiles;
+    if (ai >= a.A) return;
+    const int b0 = (lin % ntiles) * TB;
     const cx<T>* tw = (const cx<T>*)a.tw;
     const int hin = a.nin / 2, hout = a.nout / 2;
     for (int jb = 0; jb < a.njobs; jb++) {
@@ -872,8 +880,11 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
 {
     constexpr int TB = strided_tb<T, L>();
     constexpr int NT = 256;
-    dim3 grid((a.B + TB - 1) / TB, A, 1);
-    hipLaunchKernelGGL((k_strided<TIN, TOUT, T, L, TB, NT, DIR>), grid, dim3(NT), 0, st, a);
+    const int64_t blocks = (int64_t)((a.B + TB - 1) / TB) * A;
+    StridedArgs b = a;
+    b.A = A;
+    dim3 grid((unsigned)((blocks + 7) / 8 * 8), 1, 1);
+    hipLaunchKernelGGL((k_strided<TIN, TOUT, T, L, TB, NT, DIR>), grid, dim3(NT), 0, st, b);
     return hipGetLastError();
 }
 
