@@ -163,7 +163,7 @@ struct spns_ctx {
     double* forced = nullptr;
     double* ft = nullptr;
     int grid_epi = 0, grid_z = 0;
-    int zkernel = 0;  // z-pass variant (SPNS_ZPASS=block selects the block kernel)
+    int zkernel = ZK_STAGED;  // z-pass kernel (environment SPNS_ZPASS=staged|block|wave overrides)
     hipGraphExec_t graph = nullptr;
     bool graph_ready = false;
     std::string err;
@@ -216,8 +216,11 @@ struct spns_ctx {
         sa = cfg.isSelfAdaptive != 0;
         nfwd = les ? 9 : 3;
 
-        zkernel = cfg.zpass_variant == 2 ? 0 : 1;  // default: block kernel
-        if (const char* zk = std::getenv("SPNS_ZPASS")) zkernel = std::string(zk) == "wave" ? 0 : 1;
+        zkernel = cfg.zpass_variant == 1 ? ZK_BLOCK : (cfg.zpass_variant == 2 ? ZK_WAVE : ZK_STAGED);
+        if (const char* zk = std::getenv("SPNS_ZPASS")) {
+            const std::string v(zk);
+            zkernel = v == "wave" ? ZK_WAVE : (v == "block" ? ZK_BLOCK : ZK_STAGED);
+        }
         HIPCHK(hipSetDevice(cfg.device));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ev0));
@@ -239,7 +242,15 @@ struct spns_ctx {
         alloc((void**)&scal, sizeof(DevScalars));
         grid_epi = (int)std::min<int64_t>((nmodes_p + 255) / 256, 1024);
         const int64_t npen = (int64_t)My * Mx;
-        grid_z = (int)std::min<int64_t>(npen, 65536);
+        {
+            ZArgs zq{};
+            zq.kernel = zkernel;
+            zq.pitch = nk3p;
+            zq.nk3 = nk3;
+            zq.les = les;
+            grid_z = zpass_grid(Mz, prec, zq, npen);
+            if (grid_z <= 0) throw Error(SPNS_EHIP, "z pass: no launch configuration for this length");
+        }
         alloc((void**)&hist_partial, sizeof(double) * (size_t)grid_epi * nbins);
         alloc((void**)&om_partial, sizeof(double) * grid_epi);
         alloc((void**)&skew_partial, sizeof(double) * 2 * grid_z);

@@ -316,6 +316,134 @@ struct Fft {
 
 
 // ---------------------------------------------------------------------------
+// LDS-only block barrier: waits for this wave's LDS operations, then meets
+// the other waves of the block.  Unlike __syncthreads it does not wait for
+// outstanding global-memory operations, so asynchronous global->LDS copies
+// (dma16) issued earlier keep flowing across it.  The memory clobber keeps
+// the compiler from moving memory accesses across the barrier.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Asynchronous copy of 16 bytes per lane from global memory to LDS
+// (gfx950 global_load_lds_dwordx4): active lane i's bytes land at LDS byte
+// address lds_base + 16*i; lds_base must be wave-uniform.  The copy is
+// counted by vmcnt only: dma_wait() then a barrier publish it to the block.
+// Issued from inline assembly so the compiler does not serialise later LDS
+// reads of unrelated buffers behind it.  (s_nop: one wait state between the
+// M0 write and the LDS-DMA instruction.)
+__device__ __forceinline__ void dma16(const void* g, uint32_t lds_base)
+{
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_base), "v"(g)
+                 : "memory", "m0");
+}
+
+__device__ __forceinline__ void dma_wait()
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <typename P>
+__device__ __forceinline__ uint32_t lds_addr(P* p)
+{
+    return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)(p);
+}
+
+// ---------------------------------------------------------------------------
+// Batched Stockham FFT like Fft, for kernels that overlap asynchronous
+// global->LDS copies with the transform:
+//   * stages meet at lds_barrier() (no wait on global memory);
+//   * twiddles come from an LDS copy of the forward table; in FP64 each
+//     butterfly reads only w = tw[k*TWS] and forms w^r by repeated complex
+//     multiplication (error a few ulp, LDS traffic 1/(R-1) of a full read);
+//   * hook() runs once, right after the barrier that ends stage 0, when
+//     every wave has finished reading stage 0's inputs.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false>
+struct FftS {
+    using C = cx<T>;
+    static constexpr int NS = Plan<L>::n;
+    static_assert(NS >= 2, "FftS needs at least two stages");
+
+    template <int S, class Load, class Store, class Hook>
+    __device__ static __forceinline__ void stage(C* __restrict__ lds, const C* __restrict__ twl, Load& load, Store& store,
+                                                 Hook& hook)
+    {
+        constexpr int R = Plan<L>::r(S);
+        constexpr int Ns = plan_span<L, S>();
+        constexpr int LR = L / R;
+        constexpr int NBF = LR * NB;
+        constexpr int BPT = (NBF + NT - 1) / NT;
+        constexpr int TWS = L / (Ns * R);
+        const int tid = threadIdx.x;
+        C v[BPT][R];
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int idx = tid + q * NT;
+            if (NBF % NT == 0 || idx < NBF) {
+                const int b = idx % NB, j = idx / NB;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if constexpr (S == 0) v[q][r] = load(b, j + r * LR);
+                    else v[q][r] = lds[(j + r * LR) * LD + b];
+                }
+            }
+        }
+        if constexpr (S > 0 || SYNC0) lds_barrier();
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int idx = tid + q * NT;
+            if (NBF % NT == 0 || idx < NBF) {
+                const int b = idx % NB, j = idx / NB;
+                const int k = j % Ns;
+                if constexpr (S > 0) {
+                    if constexpr (sizeof(T) == 8) {
+                        C w1 = twl[k * TWS];
+                        if (DIR > 0) w1.y = -w1.y;
+                        C w = w1;
+#pragma unroll
+                        for (int r = 1; r < R; r++) {
+                            v[q][r] = cmul(v[q][r], w);
+                            if (r + 1 < R) w = cmul(w, w1);
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 1; r < R; r++) {
+                            C w = twl[r * k * TWS];
+                            if (DIR > 0) w.y = -w.y;
+                            v[q][r] = cmul(v[q][r], w);
+                        }
+                    }
+                }
+                dft<R, DIR>(v[q]);
+                const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if constexpr (S == NS - 1) store(b, base + r * Ns, v[q][r]);
+                    else lds[(base + r * Ns) * LD + b] = v[q][r];
+                }
+            }
+        }
+        if constexpr (S < NS - 1) {
+            lds_barrier();
+            if constexpr (S == 0) hook();
+            stage<S + 1>(lds, twl, load, store, hook);
+        }
+    }
+
+    // lead: start with a barrier (the buffer may still be read by a previous use)
+    template <bool LEAD, class Load, class Store, class Hook>
+    __device__ static __forceinline__ void run(C* __restrict__ lds, const C* __restrict__ twl, Load load, Store store,
+                                               Hook hook)
+    {
+        if constexpr (LEAD) lds_barrier();
+        stage<0>(lds, twl, load, store, hook);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // One-wave FFT of a single length-L column (L = 64*E).  The first radix is E,
 // so stage 0 reads points t + 64 r (r < E) of lane t: one coalesced load per
 // r.  Every later radix divides E, so the last stage leaves lane t holding

@@ -40,12 +40,14 @@ struct StridedArgs {
     const void* tw;
 };
 
+enum { ZK_WAVE = 0, ZK_BLOCK = 1, ZK_STAGED = 2 };
+
 struct ZArgs {
     void* f[9];  // X2 fields: u v w | ux vx wx | uy vy wy ; pencil offset pen*pitch
     int nk3, pitch;
     int les;
     int sa;  // also reduce the self-adaptive sum form of the CFL velocity
-    int kernel;  // 0: wave-per-pencil where a wave plan exists, 1: block kernel
+    int kernel;  // ZK_WAVE, ZK_BLOCK or ZK_STAGED
     int64_t npencil;
     double Ax, Ay, Az;  // CFL divisors (src/HIT_LES.h:56-58)
     double ct;          // (C_Smag*delta)^2 (src/HIT_LES.cpp:132)
@@ -134,6 +136,8 @@ hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipS
 hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);
 hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);
 hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st);
+int zpass_grid(int L, int prec, const ZArgs& a, int64_t npen);  // < 0: no such length / HIP error
+int zpass_kernel(int L, int prec, const ZArgs& a);              // kernel launch_zpass picks
 hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid, hipStream_t st);
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st);
 hipError_t launch_finalize(const FinArgs& a, hipStream_t st);

@@ -309,6 +309,220 @@ __global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
 
 
 // ---------------------------------------------------------------------------
+// z pass with the next pencil's inputs in flight (the default form).
+// Same arithmetic as k_zpass, restructured around HBM latency:
+//   * the nine half-spectrum source fields of a pencil are staged into LDS
+//     as [field][k] (pitch SP = pitch + pad) by asynchronous global->LDS
+//     copies (dma16), issued for pencil n+1 as soon as pencil n's first
+//     inverse stage has consumed the staging area, so the copy overlaps the
+//     rest of pencil n's transforms and products;
+//   * all barriers inside the loop are LDS-only (lds_barrier);
+//   * twiddles are read from an LDS copy of the table (FftS);
+//   * field pointers come from a small LDS table, so no per-lane indexing
+//     of kernel arguments.
+// Staging capacity is a compile-time maximum (zstage_spc); the launcher
+// checks pitch + pad against it and the LDS total against the CU's 160 KB.
+// ---------------------------------------------------------------------------
+template <int L>
+__host__ __device__ constexpr int zstage_spc()
+{
+    return (L / 3 + 1 + 15) / 16 * 16 + 2;
+}
+
+template <typename T>
+__host__ __device__ constexpr int zstage_pad()
+{
+    return sizeof(T) == 8 ? 1 : 2;  // odd pitch (FP64) / even pitch for 16-byte pairs (FP32)
+}
+
+template <typename T, int L, int NT, int MODE>
+__host__ __device__ constexpr size_t zstage_lds_bytes()
+{
+    return sizeof(cx<T>) * ((size_t)L * (MODE == 0 ? 6 : 2) + (size_t)(MODE == 0 ? 9 : 3) * zstage_spc<L>() + L) +
+           sizeof(double) * NT + 2 * 9 * sizeof(void*);
+}
+
+template <typename T, int L, int NT, int MODE>
+__host__ __device__ constexpr bool zstage_fits()
+{
+    return zstage_lds_bytes<T, L, NT, MODE>() <= 160 * 1024 - 512;
+}
+
+template <typename T, int L, int NT, int MODE>
+__global__ __launch_bounds__(NT) void k_zstage(ZArgs a)
+{
+    constexpr int LD = (MODE == 0) ? 6 : 2;
+    constexpr int NSRC = (MODE == 0) ? 9 : 3;
+    constexpr int SPC = zstage_spc<L>();
+    constexpr int NW = NT / 64;
+    constexpr int EPS = 16 / (int)sizeof(cx<T>);  // elements per 16-byte copy slot
+    __shared__ __align__(16) cx<T> fbuf[L * LD];
+    __shared__ __align__(16) cx<T> stg[NSRC * SPC];
+    __shared__ __align__(16) cx<T> twl[L];
+    __shared__ double red[NT];
+    __shared__ const cx<T>* srcp[9];
+    __shared__ cx<T>* dstp[9];
+    const int t = threadIdx.x;
+    const int nk = a.nk3, pitch = a.pitch;
+    const int SP = pitch + zstage_pad<T>();
+    const int nslot = NSRC * SP / EPS;
+    T mu = (T)0, mv = (T)0, mw = (T)0;
+    double vmax_sa = 0.0, s2 = 0.0, s3 = 0.0;
+    const double iAx = a.Ax, iAy = a.Ay, iAz = a.Az;
+    const bool sa = a.sa;
+
+    if (t < 9) {
+        srcp[t] = (const cx<T>*)a.f[t];
+        dstp[t] = (cx<T>*)a.f[t];
+    }
+    for (int i = t; i < L; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
+    __syncthreads();
+
+    // copy slot s (16 bytes) holds staging elements s*EPS .. s*EPS+EPS-1,
+    // element e = field f * SP + k; slots with k >= pitch (the pad) are skipped
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+    const uint32_t stg_base = lds_addr(stg);
+    auto issue = [&](int64_t pen) {
+        const int64_t off = pen * (int64_t)pitch;
+        for (int m = wave; m * 64 < nslot; m += NW) {
+            const int s = m * 64 + lane;
+            const int e = s * EPS;
+            const int f = e / SP, k = e - f * SP;
+            if (s < nslot && k < pitch) dma16(srcp[f] + off + k, stg_base + (uint32_t)m * 1024u);
+        }
+    };
+
+    // pair b of the inverse: logical fields (2b, 2b+1) of
+    //   u v w ux uy uz vx vy vz wx wy wz
+    // from source fields (u v w | ux vx wx | uy vy wy); d_z = i k3 * (u v w)
+    //   A: u w uy vx vz wy  -> 0 2 6 4 1 8   (d_z for vz)
+    //   B: v ux uz vy wx wz -> 1 3 0 7 5 2   (d_z for uz, wz)
+    constexpr uint32_t SRC_A = 0x814620u, SRC_B = 0x257031u, DZ_A = 0x10u, DZ_B = 0x24u;
+
+    int64_t pen = blockIdx.x;
+    if (pen < a.npencil) issue(pen);
+    dma_wait();
+    lds_barrier();
+    for (; pen < a.npencil; pen += gridDim.x) {
+        const int64_t off = pen * (int64_t)pitch;
+        const int64_t nxt = pen + gridDim.x;
+        constexpr int NPI = (MODE == 0) ? 6 : 2;
+        auto load_inv = [&](int b, int p) -> cx<T> {
+            const bool lo = p < nk, hi = p > L - nk;
+            if (!(lo || hi)) return cx<T>{(T)0, (T)0};
+            const int k = lo ? p : L - p;
+            const T kk = (T)k;
+            const int fa = (SRC_A >> (4 * b)) & 15, fb = (SRC_B >> (4 * b)) & 15;
+            cx<T> A = stg[fa * SP + k];
+            if ((DZ_A >> b) & 1) A = cx<T>{-kk * A.y, kk * A.x};
+            cx<T> B = cx<T>{(T)0, (T)0};
+            if (MODE == 0 || b == 0) {
+                B = stg[fb * SP + k];
+                if ((DZ_B >> b) & 1) B = cx<T>{-kk * B.y, kk * B.x};
+            }
+            const T s = lo ? (T)1 : (T)-1;
+            const T ay = (p == 0) ? (T)0 : s * A.y;
+            const T by = (p == 0) ? (T)0 : s * B.y;
+            return cx<T>{A.x - by, ay + B.x};
+        };
+        auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * LD + b] = v; };
+        auto prefetch = [&]() {
+            if (nxt < a.npencil) issue(nxt);
+        };
+        auto none = [&]() {};
+        FftS<T, L, NPI, NT, LD, +1>::template run<false>(fbuf, twl, load_inv, store_lds, prefetch);
+        lds_barrier();
+
+        // -------- physical space
+        for (int n = t; n < L; n += NT) {
+            cx<T>* row = &fbuf[n * LD];
+            const T u = row[0].x, v = row[0].y, w = row[1].x;
+            mu = fmax(mu, fabs(u));
+            mv = fmax(mv, fabs(v));
+            mw = fmax(mw, fabs(w));
+            if (sa) vmax_sa = fmax(vmax_sa, fabs((double)u) / iAx + fabs((double)v) / iAy + fabs((double)w) / iAz);  // src/HIT_LES.cpp:377
+            if constexpr (MODE == 1) {
+                if (a.phys) {
+                    const int64_t P = a.npencil * L;
+                    T* o = (T*)a.phys;
+                    o[pen * L + n] = u;
+                    o[P + pen * L + n] = v;
+                    o[2 * P + pen * L + n] = w;
+                }
+            } else {
+                const T ux = row[1].y, uy = row[2].x, uz = row[2].y;
+                const T vx = row[3].x, vy = row[3].y, vz = row[4].x;
+                const T wx = row[4].y, wy = row[5].x, wz = row[5].y;
+                s2 += (double)ux * (double)ux;
+                s3 += (double)ux * (double)ux * (double)ux;
+                // advective form (src/HIT_LES.cpp:118-120)
+                const T cxv = u * ux + v * uy + w * uz;
+                const T cyv = u * vx + v * vy + w * vz;
+                const T czv = u * wx + v * wy + w * wz;
+                if (a.les) {
+                    // Smagorinsky (src/HIT_LES.cpp:130-154)
+                    const T ct = (T)a.ct;
+                    T ne = (T)2.0 * ux * ux;
+                    ne += (T)2.0 * vy * vy;
+                    ne += (T)2.0 * wz * wz;
+                    ne += (uy + vx) * (uy + vx);
+                    ne += (uz + wx) * (uz + wx);
+                    ne += (vz + wy) * (vz + wy);
+                    ne = ct * sqrt(ne);
+                    row[0] = cx<T>{cxv, cyv};
+                    row[1] = cx<T>{czv, (T)(-2.0) * ne * ux};
+                    row[2] = cx<T>{(T)(-1.0) * ne * (uy + vx), (T)(-2.0) * ne * vy};
+                    row[3] = cx<T>{(T)(-1.0) * ne * (uz + wx), (T)(-1.0) * ne * (vz + wy)};
+                    row[4] = cx<T>{(T)(-2.0) * ne * wz, (T)0};
+                } else {
+                    row[0] = cx<T>{cxv, cyv};
+                    row[1] = cx<T>{czv, (T)0};
+                }
+            }
+        }
+        if constexpr (MODE == 0) {
+            auto load_lds = [&](int b, int p) -> cx<T> { return fbuf[p * LD + b]; };
+            if (a.les) FftS<T, L, 5, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none);
+            else FftS<T, L, 2, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none);
+            lds_barrier();
+            dma_wait();  // the next pencil's sources (this wave's copies) have landed
+            const int npairs = a.les ? 5 : 2;
+            const int nfields = a.les ? 9 : 3;
+            for (int idx = t; idx < npairs * nk; idx += NT) {
+                const int q = idx / nk, k = idx - q * nk;
+                const cx<T> Z = fbuf[k * LD + q];
+                const cx<T> Zc = fbuf[((L - k) % L) * LD + q];
+                const T h = (T)0.5;
+                const cx<T> A = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
+                const cx<T> B = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
+                dstp[2 * q][off + k] = A;
+                if (2 * q + 1 < nfields) dstp[2 * q + 1][off + k] = B;
+            }
+        } else {
+            dma_wait();
+        }
+        lds_barrier();  // every wave's copies are in; fbuf is free again
+    }
+    const double bu = block_max<NT>((double)mu, red);
+    const double bv = block_max<NT>((double)mv, red);
+    const double bw = block_max<NT>((double)mw, red);
+    const double vmax = fmax(bu / iAx, fmax(bv / iAy, bw / iAz));  // src/HIT_LES.cpp:393
+    if (sa) vmax_sa = block_max<NT>(vmax_sa, red);
+    if (MODE == 0) {
+        s2 = block_sum<NT>(s2, red);
+        s3 = block_sum<NT>(s3, red);
+    }
+    if (t == 0) {
+        atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
+        atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
+        if (MODE == 0) {
+            a.skew_partial[2 * blockIdx.x] = s2;
+            a.skew_partial[2 * blockIdx.x + 1] = s3;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // z pass, one wave per pencil (DNS and velocity-only modes, L = 64*E).
 // Lane t owns the physical points z = t + 64 m; the twelve fields arrive pair
 // by pair from WaveFft and the products u.grad(u) accumulate in registers in
@@ -953,17 +1167,78 @@ hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, c
 
 #if SPNS_PART == 0 || SPNS_PART == 3
 template <typename T, int L, int MODE>
+static bool zstage_ok(const ZArgs& a)
+{
+    constexpr int NT = zpass_nt<L>();
+    if constexpr (!zstage_fits<T, L, NT, MODE>()) return false;
+    return a.pitch + zstage_pad<T>() <= zstage_spc<L>() && a.nk3 <= a.pitch;
+}
+
+template <typename T, int L, int MODE>
 static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
 {
     if constexpr (has_wave_plan<L>::value) {
-        if ((!a.les || MODE == 1) && a.kernel == 0) {
+        if ((!a.les || MODE == 1) && a.kernel == ZK_WAVE) {
             hipLaunchKernelGGL((k_zwave<T, L, MODE>), dim3(grid), dim3(256), 0, st, a);
             return hipGetLastError();
         }
     }
     constexpr int NT = zpass_nt<L>();
+    if constexpr (zstage_fits<T, L, NT, MODE>()) {
+        if (a.kernel == ZK_STAGED && zstage_ok<T, L, MODE>(a)) {
+            hipLaunchKernelGGL((k_zstage<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((k_zpass<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
     return hipGetLastError();
+}
+
+// Grid of the z pass: the staged kernel runs persistent blocks (as many as
+// fit on the device at once), the others one block per pencil up to 65536.
+template <typename T, int L>
+static int zpass_grid_L(const ZArgs& a, int64_t npen)
+{
+    if (a.kernel == ZK_STAGED && zstage_ok<T, L, 0>(a)) {
+        constexpr int NT = zpass_nt<L>();
+        if constexpr (zstage_fits<T, L, NT, 0>()) {
+            int dev = 0, ncu = 0, nb = 0;
+            if (hipGetDevice(&dev) != hipSuccess) return -1;
+            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zstage<T, L, NT, 0>, NT, 0) != hipSuccess) return -1;
+            const int64_t g = (int64_t)ncu * (nb > 0 ? nb : 1);
+            return (int)(npen < g ? npen : g);
+        }
+    }
+    return (int)(npen < 65536 ? npen : 65536);
+}
+
+int zpass_grid(int L, int prec, const ZArgs& a, int64_t npen)
+{
+#define X(LL)                                                                                          \
+    if (L == LL) return prec == PREC_FP64 ? zpass_grid_L<double, LL>(a, npen) : zpass_grid_L<float, LL>(a, npen);
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return -1;
+}
+
+// Kernel a z pass with these arguments runs: ZK_STAGED falls back to
+// ZK_BLOCK where the staged kernel's LDS does not fit.
+template <typename T, int L>
+static int zpass_kernel_L(const ZArgs& a)
+{
+    if (a.kernel == ZK_WAVE && has_wave_plan<L>::value && !a.les) return ZK_WAVE;
+    if (a.kernel == ZK_STAGED && zstage_ok<T, L, 0>(a)) return ZK_STAGED;
+    return ZK_BLOCK;
+}
+
+int zpass_kernel(int L, int prec, const ZArgs& a)
+{
+#define X(LL)                                                                                        \
+    if (L == LL) return prec == PREC_FP64 ? zpass_kernel_L<double, LL>(a) : zpass_kernel_L<float, LL>(a);
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return -1;
 }
 
 hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st)
