@@ -124,6 +124,19 @@ def cpu_baseline(n_cpu, threads):
     return dt, lib().orc_num_threads()
 
 
+def copy_calibration(mib):
+    """A plain device copy of known size: the reference reading for the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) of a profiled run."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +148,12 @@ def main():
     ap.add_argument("--cpu-N", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one GPU; the step shards as a slab decomposition only in design "
               "(see DESIGN.md).  Use --gpus 1.", file=sys.stderr)
