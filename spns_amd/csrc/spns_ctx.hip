@@ -73,13 +73,23 @@ void* upload_twiddles(int L, int prec)
 inline int ffreq(int a, int M) { return (a <= M / 2) ? a : a - M; }  // src/util.h:67
 
 // Checks every address a strided launch can touch against the buffers.
-void check_strided(const StridedArgs& a, int A, const int64_t* in_cap, const int64_t* out_cap)
+void check_strided(StridedArgs& a, int A, const int64_t* in_cap, const int64_t* out_cap, int tile)
 {
+    static const bool plain = [] {
+        const char* v = std::getenv("SPNS_STRIDED");
+        return v && std::string(v) == "plain";
+    }();
+    a.padded_ok = tile > 0 && !plain;
     for (int j = 0; j < a.njobs; j++) {
         int64_t imax = (int64_t)(A - 1) * a.in_a_stride + (int64_t)(a.nin - 1) * a.in_row_stride + (a.B - 1);
         int64_t omax = (int64_t)(A - 1) * a.out_a_stride + (int64_t)(a.nout - 1) * a.out_row_stride + (a.B - 1);
         if (imax >= in_cap[j] || omax >= out_cap[j] || a.nin < 1 || a.nout < 1 || a.B < 1)
             throw Error(SPNS_EINVAL, "strided pass would address outside its buffers");
+        if (tile > 0) {
+            // the staged kernel copies whole column tiles: columns up to round_up(B, tile) - 1
+            const int64_t bpad = ((int64_t)a.B + tile - 1) / tile * tile;
+            if (imax - (a.B - 1) + (bpad - 1) >= in_cap[j]) a.padded_ok = 0;
+        }
     }
 }
 
@@ -105,7 +115,7 @@ struct spns_fft {
         a.in_a_stride = (int64_t)Mx * nk; a.out_a_stride = (int64_t)Mx * nk;
         a.tw = twx;
         int64_t cap = (int64_t)My * Mx * nk;
-        check_strided(a, My, &cap, &cap);
+        check_strided(a, My, &cap, &cap, 0);
         HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, dir, a, My, st));
     }
     void fft_y(void* c, int dir)
@@ -118,7 +128,7 @@ struct spns_fft {
         a.in_row_stride = (int64_t)Mx * nk; a.out_row_stride = (int64_t)Mx * nk;
         a.tw = twy;
         int64_t cap = (int64_t)My * Mx * nk;
-        check_strided(a, 1, &cap, &cap);
+        check_strided(a, 1, &cap, &cap, 0);
         HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, dir, a, 1, st));
     }
     void z(const void* in, void* out, int dir)
@@ -311,7 +321,7 @@ struct spns_ctx {
         a.tw = twy;
         int64_t icap[9], ocap[9];
         for (int j = 0; j < n; j++) { icap[j] = nmodes_p; ocap[j] = x1_slot; }
-        check_strided(a, nk1, icap, ocap);
+        check_strided(a, nk1, icap, ocap, strided_tile(My, prec));
         HIPCHK(launch_strided(My, prec, 1, prec == PREC_FP64, +1, a, nk1, st));
     }
 
@@ -334,7 +344,7 @@ struct spns_ctx {
         a.tw = twx;
         int64_t icap[9], ocap[9];
         for (int j = 0; j < n; j++) { icap[j] = x1_slot; ocap[j] = x2_slot; }
-        check_strided(a, My, icap, ocap);
+        check_strided(a, My, icap, ocap, strided_tile(Mx, prec));
         HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, +1, a, My, st));
     }
 
@@ -370,7 +380,7 @@ struct spns_ctx {
         a.tw = twx;
         int64_t icap[9], ocap[9];
         for (int j = 0; j < nfwd; j++) { icap[j] = x2_slot; ocap[j] = x1_slot; }
-        check_strided(a, My, icap, ocap);
+        check_strided(a, My, icap, ocap, strided_tile(Mx, prec));
         HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, My, st));
     }
 
@@ -385,7 +395,7 @@ struct spns_ctx {
         a.tw = twy;
         int64_t icap[9], ocap[9];
         for (int j = 0; j < nfwd; j++) { icap[j] = x1_slot; ocap[j] = nmodes_p; }
-        check_strided(a, nk1, icap, ocap);
+        check_strided(a, nk1, icap, ocap, strided_tile(My, prec));
         if ((int64_t)nfwd * nmodes_p > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
         HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, nk1, st));
     }

@@ -36,6 +36,7 @@ struct StridedArgs {
     int nin;   // stored input rows (odd, wrapped order) or L
     int nout;  // stored output rows (odd, wrapped order) or L
     int A;     // outer extent (set by the launcher)
+    int padded_ok;  // column tiles rounded up to the tile width stay inside every input buffer
     int64_t in_a_stride, in_row_stride, out_a_stride, out_row_stride;  // in complex elements
     const void* tw;
 };

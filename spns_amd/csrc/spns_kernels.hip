@@ -97,6 +97,131 @@ __global__ __launch_bounds__(NT) void k_strided(StridedArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Strided batched pass with the next tile's input in flight (the default
+// form of k_strided).  Persistent blocks walk work items (a, job, column
+// tile); item order puts neighbouring column tiles of one row on the same
+// XCD at the same time (they share cache lines).  Per item:
+//   stage 0 reads the staged input tile [row][TB] (TIN) from LDS, with the
+//   pruning and the i k derivative of k_strided; after its barrier the
+//   block issues the asynchronous global->LDS copy (dma16) of the next
+//   item's tile; the remaining stages run on fbuf; the last stage writes
+//   fbuf; then each wave waits for its own copies and the block writes the
+//   output tile from fbuf to global memory, so the wait never covers the
+//   item's fresh stores.
+// Requires the column tile rounded up to TB to stay inside the input
+// buffers (StridedArgs::padded_ok, checked by the host).
+// ---------------------------------------------------------------------------
+template <int L, int DIR>
+__host__ __device__ constexpr int sstage_rows()
+{
+    return DIR > 0 ? (2 * L) / 3 + 2 : L;  // pruned inverse input: N + 1 rows of M = 3N/2
+}
+
+template <int L, int TB>
+__host__ __device__ constexpr int sstage_nt()
+{
+    int m = 0;
+    for (int s = 0; s < Plan<L>::n; s++) {
+        const int b = (L / Plan<L>::r(s)) * TB;
+        m = b > m ? b : m;
+    }
+    m = (m + 63) / 64 * 64;
+    return m > 1024 ? 1024 : (m < 64 ? 64 : m);
+}
+
+template <typename TIN, typename T, int L, int TB, int DIR>
+__host__ __device__ constexpr size_t sstage_lds_bytes()
+{
+    return sizeof(cx<T>) * (size_t)L * (TB + 1) + sizeof(cx<TIN>) * (size_t)sstage_rows<L, DIR>() * TB;
+}
+
+template <typename TIN, typename T, int L, int TB, int DIR>
+__host__ __device__ constexpr bool sstage_fits()
+{
+    return sstage_lds_bytes<TIN, T, L, TB, DIR>() <= 160 * 1024 - 512 && (TB * sizeof(cx<TIN>)) % 16 == 0;
+}
+
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR>
+__global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
+{
+    constexpr int SROWS = sstage_rows<L, DIR>();
+    constexpr int SPR = TB * (int)sizeof(cx<TIN>) / 16;  // 16-byte copy slots per staged row
+    constexpr int EPS = 16 / (int)sizeof(cx<TIN>);        // input elements per slot
+    constexpr int NW = NT / 64;
+    __shared__ __align__(16) cx<T> fbuf[L * TB];
+    __shared__ __align__(16) cx<TIN> stg[SROWS * TB];
+    __shared__ __align__(16) cx<T> twl[L];
+    const int t = threadIdx.x;
+    const int hin = a.nin / 2, hout = a.nout / 2;
+    const int ntiles = (a.B + TB - 1) / TB;
+    const int64_t per_a = (int64_t)a.njobs * ntiles;
+    const int64_t nitems = (int64_t)a.A * per_a;
+    const int G = gridDim.x;  // a multiple of 8
+    const int64_t lin = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+
+    for (int i = t; i < L; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
+    __syncthreads();
+
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+    const uint32_t stg_base = lds_addr(stg);
+    const int nslot = a.nin * SPR;
+    auto issue = [&](int64_t it) {
+        const int64_t ai = it / per_a;
+        const int r = (int)(it - ai * per_a);
+        const int jb = r / ntiles, b0 = (r - jb * ntiles) * TB;
+        const cx<TIN>* in = (const cx<TIN>*)a.job[jb].in + ai * a.in_a_stride + b0;
+        for (int m = wave; m * 64 < nslot; m += NW) {
+            const int s = m * 64 + lane;
+            const int row = s / SPR, part = s - row * SPR;
+            if (s < nslot) dma16(in + (int64_t)row * a.in_row_stride + part * EPS, stg_base + (uint32_t)m * 1024u);
+        }
+    };
+
+    int64_t it = lin;
+    if (it < nitems) issue(it);
+    dma_wait();
+    lds_barrier();
+    for (; it < nitems; it += G) {
+        const int64_t ai = it / per_a;
+        const int r = (int)(it - ai * per_a);
+        const int jb = r / ntiles, b0 = (r - jb * ntiles) * TB;
+        const int deriv = a.job[jb].deriv;
+        const int64_t nxt = it + G;
+        auto load = [&](int b, int p) -> cx<T> {
+            int i, k;
+            if (a.nin == L) { i = p; k = (p <= L / 2) ? p : p - L; }
+            else if (p <= hin) { i = p; k = p; }
+            else if (p >= L - hin) { i = a.nin - (L - p); k = p - L; }
+            else return cx<T>{(T)0, (T)0};
+            const cx<TIN> v = stg[i * TB + b];
+            if (deriv) {
+                // i k u, formed in the input precision (src/HIT_LES.cpp:37)
+                const TIN kk = (TIN)k;
+                return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
+            }
+            return cx<T>{(T)v.x, (T)v.y};
+        };
+        auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * TB + b] = v; };
+        auto prefetch = [&]() {
+            if (nxt < nitems) issue(nxt);
+        };
+        FftS<T, L, TB, NT, TB, DIR>::template run<false>(fbuf, twl, load, store_lds, prefetch);
+        lds_barrier();
+        dma_wait();  // the next item's tile (this wave's copies) has landed
+        cx<TOUT>* out = (cx<TOUT>*)a.job[jb].out + ai * a.out_a_stride + b0;
+        for (int idx = t; idx < a.nout * TB; idx += NT) {
+            const int i = idx / TB, b = idx - i * TB;
+            const int p = (a.nout == L || i <= hout) ? i : L - (a.nout - i);
+            if (b0 + b < a.B) {
+                const cx<T> v = fbuf[p * TB + b];
+                out[(int64_t)i * a.out_row_stride + b] = cx<TOUT>{(TOUT)v.x, (TOUT)v.y};
+            }
+        }
+        lds_barrier();  // every wave's copies are in; fbuf is free again
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Block reductions
 // ---------------------------------------------------------------------------
 template <int NT>
@@ -1093,10 +1218,30 @@ template <typename TIN, typename TOUT, typename T, int L, int DIR>
 static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
 {
     constexpr int TB = strided_tb<T, L>();
-    constexpr int NT = 256;
-    const int64_t blocks = (int64_t)((a.B + TB - 1) / TB) * A;
     StridedArgs b = a;
     b.A = A;
+    if constexpr (sstage_fits<TIN, T, L, TB, DIR>()) {
+        if (a.padded_ok && a.nin <= sstage_rows<L, DIR>()) {
+            constexpr int NT = sstage_nt<L, TB>();
+            static const int occ = [] {
+                int n = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sstage<TIN, TOUT, T, L, TB, NT, DIR>, NT, 0) != hipSuccess) n = 1;
+                return n > 0 ? n : 1;
+            }();
+            int dev = 0, ncu = 0;
+            hipError_t e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e != hipSuccess) return e;
+            const int64_t items = (int64_t)A * a.njobs * ((a.B + TB - 1) / TB);
+            int64_t g = (int64_t)ncu * occ;
+            if (items < g) g = items;
+            g = (g + 7) / 8 * 8;
+            hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR>), dim3((unsigned)g), dim3(NT), 0, st, b);
+            return hipGetLastError();
+        }
+    }
+    constexpr int NT = 256;
+    const int64_t blocks = (int64_t)((a.B + TB - 1) / TB) * A;
     dim3 grid((unsigned)((blocks + 7) / 8 * 8), 1, 1);
     hipLaunchKernelGGL((k_strided<TIN, TOUT, T, L, TB, NT, DIR>), grid, dim3(NT), 0, st, b);
     return hipGetLastError();
