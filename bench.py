@@ -107,21 +107,52 @@ def load_traffic(kernel, workload):
     return d.get("bytes_per_launch")
 
 
-def cpu_baseline(n_cpu, threads):
-    """One step of the same forced-DNS setup at N=n_cpu on the host (oracle)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_cpu, threads, steps):
+    """Median wall time of `steps` steps of the same forced-DNS setup at
+    N=n_cpu on the host (the FP64 C oracle, 1 process x OpenMP threads)."""
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    from oracle.oracle import Oracle
+    from oracle.oracle import Oracle, lib
     import numpy as np
     o = Oracle(n_cpu, nu=2e-4, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2)
     rng = np.random.default_rng(0)
     st = (rng.standard_normal((3, o.nmodes)) + 1j * rng.standard_normal((3, o.nmodes))) * 1e-3
     o.set_state(st)
     o.finish_init(False)
-    t0 = time.perf_counter()
-    o.step(1)
-    dt = time.perf_counter() - t0
-    from oracle.oracle import lib
-    return dt, lib().orc_num_threads()
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.step(1)
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], ts, lib().orc_num_threads()
+
+
+def run_ensemble(n, seed):
+    """The serving mix (Poisson arrivals at 20 req/s, mixed grids, DNS/LES,
+    viscosities and rotation rates, 100-2000 steps each, FP32-split), first n
+    requests: throughput and completion-latency percentiles."""
+    from spns_amd import serve
+    reqs = serve.draw_requests(n, rate=20.0, seed=seed, steps=(100, 2000))
+    srv = serve.Server(tuner=serve.Tuner())
+    stats, done = srv.run(reqs)
+    steps = sum(r.steps for r in reqs)
+    out = dict(stats)
+    out.update({"arrival_rate_per_s": 20.0, "seed": seed, "total_steps": steps,
+                "grids": sorted(set("x".join(map(str, r.grid)) for r in reqs)),
+                "les_fraction": sum(r.C_Smag > 0 for r in reqs) / max(1, len(reqs)),
+                "zpass_variant_choice": {"%s" % (k,): v for k, v in srv.tuner.choice.items()}})
+    return out
 
 
 def copy_calibration(mib):
@@ -146,6 +177,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--prof-steps", type=int, default=3)
     ap.add_argument("--cpu-N", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--ensemble-requests", type=int, default=32,
+                    help="requests of the serving mix to run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -217,16 +251,25 @@ def main():
             "reference": "FP64 plain GPU form (which matches the FP64 oracle to 1e-12 at test sizes)",
         }
 
+    ens = None
+    if a.ensemble_requests > 0:
+        print("bench: ensemble mix, %d requests" % a.ensemble_requests, file=sys.stderr, flush=True)
+        ens = run_ensemble(a.ensemble_requests, a.seed)
+        print("bench: ensemble %.1f s wall, p50 %.2f s, p99 %.2f s" % (ens["wall_s"], ens["latency_p50_s"],
+              ens["latency_p99_s"]), file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        print("bench: cpu baseline, oracle 1 step at N=%d" % a.cpu_N, file=sys.stderr, flush=True)
-        dt, used = cpu_baseline(a.cpu_N, threads)
+        print("bench: cpu baseline, oracle %d steps at N=%d" % (a.cpu_steps, a.cpu_N), file=sys.stderr, flush=True)
+        dt, ts, used = cpu_baseline(a.cpu_N, threads, a.cpu_steps)
         Mc = dims(a.cpu_N)[0]
         scale = (M ** 3 * math.log2(M ** 3)) / (Mc ** 3 * math.log2(Mc ** 3))
         cpu = {"value": 1.0 / (dt * scale), "unit": "steps/s", "cores": used, "kind": "port",
-               "sample": "1 step of the same forced-DNS setup at N=%d (FFT grid %d^3) took %.2f s on %d threads; "
-                         "value scaled to N=%d by the P*log2(P) ratio %.2f" % (a.cpu_N, Mc, dt, used, N, scale),
+               "sample": "%d steps of the same forced-DNS setup at N=%d (FFT grid %d^3), median %.2f s/step "
+                         "(%s), 1 process x %d OpenMP threads on %s; value scaled to N=%d by the P*log2(P) "
+                         "ratio %.2f" % (a.cpu_steps, a.cpu_N, Mc, dt, ", ".join("%.2f" % t for t in ts), used,
+                                         cpu_model(), N, scale),
                "measured_steps_per_s_at_sample": 1.0 / dt}
 
     line = {
@@ -251,6 +294,7 @@ def main():
         "fp32_split_error": split_err,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(line))
 

@@ -332,12 +332,25 @@ __device__ __forceinline__ void lds_barrier()
 // address lds_base + 16*i; lds_base must be wave-uniform.  The copy is
 // counted by vmcnt only: dma_wait() then a barrier publish it to the block.
 // Issued from inline assembly so the compiler does not serialise later LDS
-// reads of unrelated buffers behind it.  (s_nop: one wait state between the
-// M0 write and the LDS-DMA instruction.)
+// reads of unrelated buffers behind it.  M0 is reserved to the compiler, so
+// the sequence saves and restores it; s_nop gives the one wait state the
+// LDS-DMA instruction needs after an M0 write.
 __device__ __forceinline__ void dma16(const void* g, uint32_t lds_base)
 {
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_base), "v"(g)
-                 : "memory", "m0");
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %2, off\n\ts_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "s"(lds_base), "v"(g)
+        : "memory");
+}
+
+// An opaque copy of v: the compiler cannot see through it, so values derived
+// from it are not hoisted out of the enclosing loop.
+__device__ __forceinline__ int opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 __device__ __forceinline__ void dma_wait()
@@ -369,7 +382,7 @@ struct FftS {
 
     template <int S, class Load, class Store, class Hook>
     __device__ static __forceinline__ void stage(C* __restrict__ lds, const C* __restrict__ twl, Load& load, Store& store,
-                                                 Hook& hook)
+                                                 Hook& hook, const int tid)
     {
         constexpr int R = Plan<L>::r(S);
         constexpr int Ns = plan_span<L, S>();
@@ -377,7 +390,6 @@ struct FftS {
         constexpr int NBF = LR * NB;
         constexpr int BPT = (NBF + NT - 1) / NT;
         constexpr int TWS = L / (Ns * R);
-        const int tid = threadIdx.x;
         C v[BPT][R];
 #pragma unroll
         for (int q = 0; q < BPT; q++) {
@@ -429,17 +441,20 @@ struct FftS {
         if constexpr (S < NS - 1) {
             lds_barrier();
             if constexpr (S == 0) hook();
-            stage<S + 1>(lds, twl, load, store, hook);
+            stage<S + 1>(lds, twl, load, store, hook, tid);
         }
     }
 
-    // lead: start with a barrier (the buffer may still be read by a previous use)
+    // LEAD: start with a barrier (the buffer may still be read by a previous
+    // use).  tid is the thread's index in the block; callers that loop may
+    // pass it through an opaque copy so the per-thread index arithmetic is
+    // recomputed per call rather than held in registers across the loop.
     template <bool LEAD, class Load, class Store, class Hook>
     __device__ static __forceinline__ void run(C* __restrict__ lds, const C* __restrict__ twl, Load load, Store store,
-                                               Hook hook)
+                                               Hook hook, const int tid)
     {
         if constexpr (LEAD) lds_barrier();
-        stage<0>(lds, twl, load, store, hook);
+        stage<0>(lds, twl, load, store, hook, tid);
     }
 };
 

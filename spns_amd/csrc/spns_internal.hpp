@@ -132,7 +132,6 @@ bool supported_length(int L);
 hipError_t launch_init_random(const RandArgs& a, hipStream_t st);
 hipError_t launch_scale_state(const ShellArgs& a, double sc, hipStream_t st);
 int strided_tile(int L, int prec);
-int zpass_threads(int L);
 hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st);
 hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);
 hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);

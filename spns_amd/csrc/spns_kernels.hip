@@ -205,7 +205,7 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
         auto prefetch = [&]() {
             if (nxt < nitems) issue(nxt);
         };
-        FftS<T, L, TB, NT, TB, DIR>::template run<false>(fbuf, twl, load, store_lds, prefetch);
+        FftS<T, L, TB, NT, TB, DIR>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
         lds_barrier();
         dma_wait();  // the next item's tile (this wave's copies) has landed
         cx<TOUT>* out = (cx<TOUT>*)a.job[jb].out + ai * a.out_a_stride + b0;
@@ -460,23 +460,38 @@ __host__ __device__ constexpr int zstage_pad()
     return sizeof(T) == 8 ? 1 : 2;  // odd pitch (FP64) / even pitch for 16-byte pairs (FP32)
 }
 
-template <typename T, int L, int NT, int MODE>
+template <int MODE>
+__host__ __device__ constexpr int zstage_ld()
+{
+    // LDS row pitch of the pair buffer, in complex elements: one more than the
+    // pairs per point, so row-strided accesses (the physical-space loop, the
+    // unpack) spread over all banks
+    return MODE == 0 ? 7 : 3;
+}
+
+template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>()>
 __host__ __device__ constexpr size_t zstage_lds_bytes()
 {
-    return sizeof(cx<T>) * ((size_t)L * (MODE == 0 ? 6 : 2) + (size_t)(MODE == 0 ? 9 : 3) * zstage_spc<L>() + L) +
+    return sizeof(cx<T>) * ((size_t)L * LD + (size_t)(MODE == 0 ? 9 : 3) * zstage_spc<L>() + L) +
            sizeof(double) * NT + 2 * 9 * sizeof(void*);
 }
 
-template <typename T, int L, int NT, int MODE>
+template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>()>
 __host__ __device__ constexpr bool zstage_fits()
 {
-    return zstage_lds_bytes<T, L, NT, MODE>() <= 160 * 1024 - 512;
+    return zstage_lds_bytes<T, L, NT, MODE, LD>() <= 160 * 1024 - 512;
 }
 
-template <typename T, int L, int NT, int MODE>
-__global__ __launch_bounds__(NT) void k_zstage(ZArgs a)
+// ZOPAQUE: route the thread index through opaque() once per pencil, so the
+// compiler recomputes the transforms' per-thread LDS addresses instead of
+// holding them in registers across the pencil loop.  In FP32 that takes the
+// kernel from 184 to 64-80 VGPRs and two blocks per CU (9.2-9.9 ms against
+// 13.7 ms at 512^3); in FP64 the LDS allows one block per CU anyway and the
+// recomputation only costs time, so it is off there.
+template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int MINW = 1, bool ZOPAQUE = (sizeof(T) == 4)>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void k_zstage(ZArgs a)
 {
-    constexpr int LD = (MODE == 0) ? 6 : 2;
+    static_assert(LD >= (MODE == 0 ? 6 : 2), "pair buffer pitch too small");
     constexpr int NSRC = (MODE == 0) ? 9 : 3;
     constexpr int SPC = zstage_spc<L>();
     constexpr int NW = NT / 64;
@@ -532,6 +547,7 @@ __global__ __launch_bounds__(NT) void k_zstage(ZArgs a)
         const int64_t off = pen * (int64_t)pitch;
         const int64_t nxt = pen + gridDim.x;
         constexpr int NPI = (MODE == 0) ? 6 : 2;
+        const int tl = ZOPAQUE ? opaque(t) : t;
         auto load_inv = [&](int b, int p) -> cx<T> {
             const bool lo = p < nk, hi = p > L - nk;
             if (!(lo || hi)) return cx<T>{(T)0, (T)0};
@@ -555,7 +571,7 @@ __global__ __launch_bounds__(NT) void k_zstage(ZArgs a)
             if (nxt < a.npencil) issue(nxt);
         };
         auto none = [&]() {};
-        FftS<T, L, NPI, NT, LD, +1>::template run<false>(fbuf, twl, load_inv, store_lds, prefetch);
+        FftS<T, L, NPI, NT, LD, +1>::template run<false>(fbuf, twl, load_inv, store_lds, prefetch, tl);
         lds_barrier();
 
         // -------- physical space
@@ -607,8 +623,8 @@ __global__ __launch_bounds__(NT) void k_zstage(ZArgs a)
         }
         if constexpr (MODE == 0) {
             auto load_lds = [&](int b, int p) -> cx<T> { return fbuf[p * LD + b]; };
-            if (a.les) FftS<T, L, 5, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none);
-            else FftS<T, L, 2, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none);
+            if (a.les) FftS<T, L, 5, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
+            else FftS<T, L, 2, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
             lds_barrier();
             dma_wait();  // the next pencil's sources (this wave's copies) have landed
             const int npairs = a.les ? 5 : 2;
@@ -904,15 +920,20 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
     const bool first = a.scal->iter == 0;
     double om = 0.0;
     const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3, nk3p = a.nk3p;
-    for (int64_t m = (int64_t)blockIdx.x * NT + threadIdx.x; m < a.nmodes; m += (int64_t)gridDim.x * NT) {
-        const int i3 = (int)(m % nk3p);
-        if (i3 >= nk3) continue;  // pitch padding
-        const int64_t r12 = m / nk3p;
-        const int i1 = (int)(r12 % nk1), i2 = (int)(r12 / nk1);
+    const double inv_nf = 1.0 / a.nf;
+    // one wave per (i2, i1) row of the compact box, lanes along i3: no
+    // per-mode index division, coalesced row reads
+    const int lane = threadIdx.x & 63;
+    const int nrow = nk1 * nk2;
+    for (int row = blockIdx.x * NW + (int)(threadIdx.x >> 6); row < nrow; row += gridDim.x * NW)
+    for (int i3 = lane; i3 < nk3; i3 += 64) {
+        const int i2 = row / nk1, i1 = row - i2 * nk1;
+        const int64_t m = (int64_t)row * nk3p + i3;
         const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2), k3 = i3;
         if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) continue;  // src/HIT_LES.cpp:867-871
         const int ik2 = k1 * k1 + k2 * k2 + k3 * k3;
         const double rad2 = (double)ik2;
+        const int K = shell_of(ik2);
         cx<double> U[3], R0[3], R1[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) { U[c] = a.uk[c][m]; R0[c] = a.R[c][m]; }
@@ -922,7 +943,7 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const cx<T> cv = ((const cx<T>*)a.conv[c])[m];
-                const double cvx = (double)cv.x / a.nf, cvy = (double)cv.y / a.nf;  // Fourier_Normalization
+                const double cvx = (double)cv.x * inv_nf, cvy = (double)cv.y * inv_nf;  // Fourier_Normalization
                 R1[c].x = -(mean ? 0.0 : cvx) - ct * U[c].x;
                 R1[c].y = -(mean ? 0.0 : cvy) - ct * U[c].y;
             }
@@ -931,8 +952,8 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
 #pragma unroll
                 for (int s = 0; s < 6; s++) {
                     const cx<T> v = ((const cx<T>*)a.conv[3 + s])[m];
-                    S[s].x = (double)v.x / a.nf;
-                    S[s].y = (double)v.y / a.nf;
+                    S[s].x = (double)v.x * inv_nf;
+                    S[s].y = (double)v.y * inv_nf;
                 }
                 // S: 11, 21, 22, 31, 32, 33 (src/HIT_LES.cpp:179-191)
                 const double K1 = k1, K2 = k2, K3 = k3;
@@ -953,7 +974,6 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
                 R1[2].x += 2 * (oy * U[0].x - ox * U[1].x); R1[2].y += 2 * (oy * U[0].y - ox * U[1].y);
             }
             if (a.force_on) {
-                const int K = shell_of(ik2);
                 if (a.kfmin <= K && K <= a.kfmax) {
 #pragma unroll
                     for (int c = 0; c < 3; c++) { R1[c].x += alpha * U[c].x; R1[c].y += alpha * U[c].y; }
@@ -968,7 +988,8 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
                 // step is the current field
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const cx<double> Ua = first ? U[c] : a.ukprev[c][m];
+                    const cx<double> Up = a.ukprev[c][m];  // read unconditionally: a select of
+                    const cx<double> Ua = first ? U[c] : Up;  // pointers would push U to scratch
                     a.ukprev[c][m] = U[c];
                     N[c].x = k05 * (2.0 * kap * U[c].x + dt * ((1.0 + kap) * R1[c].x - kap * R0[c].x) - (kap - 0.5) * Ua.x);
                     N[c].y = k05 * (2.0 * kap * U[c].y + dt * ((1.0 + kap) * R1[c].y - kap * R0[c].y) - (kap - 0.5) * Ua.y);
@@ -982,8 +1003,9 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
             }
             {
                 const double K1 = k1, K2 = k2, K3 = k3;
-                const double axr = (rad2 > 0) ? ((K1 * N[0].x + K2 * N[1].x + K3 * N[2].x) / rad2) : 0.0;
-                const double axi = (rad2 > 0) ? ((K1 * N[0].y + K2 * N[1].y + K3 * N[2].y) / rad2) : 0.0;
+                const double ir2 = (rad2 > 0) ? 1.0 / rad2 : 0.0;
+                const double axr = (K1 * N[0].x + K2 * N[1].x + K3 * N[2].x) * ir2;
+                const double axi = (K1 * N[0].y + K2 * N[1].y + K3 * N[2].y) * ir2;
                 N[0].x -= axr * K1; N[1].x -= axr * K2; N[2].x -= axr * K3;
                 N[0].y -= axi * K1; N[1].y -= axi * K2; N[2].y -= axi * K3;
             }
@@ -998,7 +1020,6 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
         const double ke = 0.5 * (U[0].x * U[0].x + U[0].y * U[0].y + U[1].x * U[1].x + U[1].y * U[1].y +
                                  U[2].x * U[2].x + U[2].y * U[2].y);
         const bool conj = (k3 != 0) && !((a.Mz % 2 == 0) && (k3 == a.Mz / 2));
-        const int K = shell_of(ik2);
         atomicAdd(&myh[K], conj ? 2 * ke : ke);
         const double en = 2 * rad2 * ke;
         om += conj ? 2 * en : en;
@@ -1311,10 +1332,18 @@ hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, c
 #endif
 
 #if SPNS_PART == 0 || SPNS_PART == 3
+// Threads of the staged z pass.  FP32 at L = 768: 12 waves (measured 9.5 ms
+// against 9.9 ms with 8 waves at 512^3); otherwise as the block kernel.
+template <typename T, int L>
+constexpr int zstage_nt()
+{
+    return (sizeof(T) == 4 && L == 768) ? 768 : zpass_nt<L>();
+}
+
 template <typename T, int L, int MODE>
 static bool zstage_ok(const ZArgs& a)
 {
-    constexpr int NT = zpass_nt<L>();
+    constexpr int NT = zstage_nt<T, L>();
     if constexpr (!zstage_fits<T, L, NT, MODE>()) return false;
     return a.pitch + zstage_pad<T>() <= zstage_spc<L>() && a.nk3 <= a.pitch;
 }
@@ -1328,13 +1357,14 @@ static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
             return hipGetLastError();
         }
     }
-    constexpr int NT = zpass_nt<L>();
-    if constexpr (zstage_fits<T, L, NT, MODE>()) {
+    constexpr int NTS = zstage_nt<T, L>();
+    if constexpr (zstage_fits<T, L, NTS, MODE>()) {
         if (a.kernel == ZK_STAGED && zstage_ok<T, L, MODE>(a)) {
-            hipLaunchKernelGGL((k_zstage<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
+            hipLaunchKernelGGL((k_zstage<T, L, NTS, MODE>), dim3(grid), dim3(NTS), 0, st, a);
             return hipGetLastError();
         }
     }
+    constexpr int NT = zpass_nt<L>();
     hipLaunchKernelGGL((k_zpass<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
     return hipGetLastError();
 }
@@ -1345,7 +1375,7 @@ template <typename T, int L>
 static int zpass_grid_L(const ZArgs& a, int64_t npen)
 {
     if (a.kernel == ZK_STAGED && zstage_ok<T, L, 0>(a)) {
-        constexpr int NT = zpass_nt<L>();
+        constexpr int NT = zstage_nt<T, L>();
         if constexpr (zstage_fits<T, L, NT, 0>()) {
             int dev = 0, ncu = 0, nb = 0;
             if (hipGetDevice(&dev) != hipSuccess) return -1;
@@ -1424,14 +1454,6 @@ hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid
     return hipErrorInvalidValue;
 }
 
-int zpass_threads(int L)
-{
-#define X(LL) \
-    if (L == LL) return zpass_nt<LL>();
-    SPNS_FOR_EACH_L(X)
-#undef X
-    return 0;
-}
 
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st)
 {

@@ -73,8 +73,9 @@ def draw_requests(n, rate=20.0, seed=0, steps=(100, 2000)):
 
 
 class Tuner:
-    """Picks the z-pass kernel variant (block or wave per pencil) of a shape
-    class by timing a few steps of each on a throwaway run."""
+    """Picks the z-pass kernel variant of a shape class (0 staged inputs,
+    1 block per pencil, 2 wave per pencil for DNS) by timing a few steps of
+    each on a throwaway run."""
 
     def __init__(self, steps=4):
         self.steps = steps
@@ -84,8 +85,8 @@ class Tuner:
         key = (args["Nx"], args["Ny"], args["Nz"], args.get("precision", 0), args.get("C_Smag", 0.0) > 0)
         if key in self.choice:
             return self.choice[key]
-        best, best_ms = 1, math.inf
-        for v in (1, 2):
+        best, best_ms = 0, math.inf
+        for v in ((0, 1) if args.get("C_Smag", 0.0) > 0 else (0, 1, 2)):
             s = Solver(zpass_variant=v, **args)
             s.init_random(seed=0)
             s.finish_init(False)
