@@ -139,6 +139,10 @@ int spns_velocity_physical(spns_ctx *ctx, double *uvw);
 int spns_recalculate_velocity_antitransform(spns_ctx *ctx);
 /* Wait for the context's stream. */
 int spns_synchronize(spns_ctx *ctx);
+/* Non-blocking: *done = 1 when every step enqueued by the last spns_step call
+ * has finished on the device, else 0.  No reference counterpart (the
+ * serving loop polls its runs with it). */
+int spns_query(spns_ctx *ctx, int32_t *done);
 /* Device time (ms) of the last spns_step call's kernels, from HIP events. */
 int spns_last_step_ms(spns_ctx *ctx, double *ms);
 /* Runs nsteps steps launched pass by pass with HIP events between the passes

@@ -54,7 +54,8 @@ EXPORTS = [
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
-    "spns_velocity_physical", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_last_step_ms", "spns_profile_passes",
+    "spns_velocity_physical", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_last_step_ms",
+    "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
 ]
@@ -96,6 +97,7 @@ def load_library(path=LIB_PATH):
         "spns_velocity_physical": (I32, [P, D]),
         "spns_recalculate_velocity_antitransform": (I32, [P]),
         "spns_synchronize": (I32, [P]),
+        "spns_query": (I32, [P, ctypes.POINTER(ctypes.c_int32)]),
         "spns_last_step_ms": (I32, [P, D]),
         "spns_profile_passes": (I32, [P, I32, D]),
         "spns_fft_create": (I32, [I32, I32, I32, I32, I32, ctypes.POINTER(P)]),
@@ -236,6 +238,12 @@ class Solver:
 
     def synchronize(self):
         self._chk(self.L.spns_synchronize(self.h))
+
+    def query(self):
+        """True when the steps enqueued by the last step() call have finished."""
+        d = ctypes.c_int32()
+        self._chk(self.L.spns_query(self.h, ctypes.byref(d)))
+        return bool(d.value)
 
     def last_step_ms(self):
         ms = ctypes.c_double()

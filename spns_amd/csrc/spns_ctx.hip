@@ -981,6 +981,21 @@ int spns_synchronize(spns_ctx* ctx)
     API_END(ctx)
 }
 
+int spns_query(spns_ctx* ctx, int32_t* done)
+{
+    API_BEGIN(ctx)
+    if (!done) throw Error(SPNS_EINVAL, "done is null");
+    const hipError_t e = hipEventQuery(ctx->ev1);
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: clear the sticky status
+        *done = 0;
+    } else {
+        HIPCHK(e);
+        *done = 1;
+    }
+    API_END(ctx)
+}
+
 int spns_profile_passes(spns_ctx* ctx, int32_t nsteps, double* ms)
 {
     API_BEGIN(ctx)

@@ -6,9 +6,10 @@ constants and rotation rates) arrive over time.  The loop
     (spns_device_bytes) fits the HBM budget left by the runs in flight;
   * keeps every admitted run resident: state, history and work buffers stay
     in HBM from admission to completion;
-  * advances all runs together, round by round: each round enqueues a chunk
-    of hipGraph-captured steps for every active run on that run's own stream
-    before waiting, so the runs' kernels overlap on the device;
+  * advances every admitted run in chunks of hipGraph-captured steps on the
+    run's own stream; the loop polls each run (spns_query) and gives it its
+    next chunk as soon as its previous one has finished, so the runs' kernels
+    overlap on the device and no run waits for another;
   * retires runs that reached their step count (collecting E(k), energy,
     dissipation and skewness), frees their memory and admits queued runs
     into the freed space while the others continue.
@@ -158,26 +159,32 @@ class Server:
             if not active:
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                 continue
-            # one round: a chunk of captured steps for every active run, each on its own stream
-            for a in active:
-                n = min(self.chunk, a["req"].steps - a["done"])
-                a["solver"].step(n)
-                a["done"] += n
-                steps_done += n
-            for a in active:
-                a["solver"].synchronize()
-            now = time.perf_counter() - t0
+            # every run whose last chunk has finished gets its next chunk (or
+            # retires); the others keep running: no run waits for another
+            progressed = False
             still = []
             for a in active:
+                s = a["solver"]
+                if a["done"] > 0 and not s.query():
+                    still.append(a)
+                    continue
                 if a["done"] >= a["req"].steps:
-                    self._finish(a, now)
+                    self._finish(a, time.perf_counter() - t0)
                     if not keep_state:
                         a["req"].result.pop("state", None)
                     used -= a["bytes"]
                     done.append(a["req"])
-                else:
-                    still.append(a)
+                    progressed = True
+                    continue
+                n = min(self.chunk, a["req"].steps - a["done"])
+                s.step(n)
+                a["done"] += n
+                steps_done += n
+                progressed = True
+                still.append(a)
             active = still
+            if not progressed:
+                time.sleep(2e-4)
         wall = time.perf_counter() - t0
         lat = np.array([r.finished - r.arrival for r in done])
         return {
