@@ -1,9 +1,11 @@
-"""Benchmark: RK (AB2) steps/s and HBM roofline at 512^3, FP32-split vs FP64.
+"""Benchmark: RK (AB2) steps/s and HBM roofline at 512^3, FP32-split vs FP64,
+plus the ensemble serving mix.
 
-Workload (the 512^3 forced DNS): Nx=Ny=Nz=512 retained
-modes, FFT grid 768^3, nu=2e-4, constant-power forcing eps=0.1 on shells
-1<=K<=2, CFL step with C_At=0.5, seeded random solenoidal field with
-E(k) ~ k^4 exp(-2 (k/k0)^2), k0=4, u_rms=1 (synthetic data).
+Workload (the 512^3 forced DNS): Nx=Ny=Nz=512 retained modes, FFT grid
+768^3, nu=2e-4, constant-power forcing eps=0.1 on shells 1<=K<=2, CFL step
+with C_At=0.5, seeded random solenoidal field with
+E(k) ~ k^4 exp(-2 (k/k0)^2), k0=4, u_rms=1 (synthetic data).  --N 1024 runs
+the 1024^3 case (FFT grid 1536^3, nu=8e-5).
 
 Timed region: K captured steps (hipGraph) with the state resident in HBM,
 bracketed by stream and device synchronisation.  `value` is the FP32-split
@@ -14,9 +16,14 @@ achieved = its algorithmic bytes per launch / its mean launch time, the
 time from HIP events on the library's stream (spns_profile_passes).
 `traffic` comes from the committed rocprofv3 PMC summary of the same
 kernel and workload when present (profiles/*.json), else null.
+`copy_peak` is a measured device copy, the practical ceiling beside 8 TB/s.
 
 cpu_baseline: the FP64 C oracle (a port of the reference step) on the
-host's cores, one step of the same forced-DNS setup at a bounded size.
+host's cores, the median of a few steps of the same forced-DNS setup at a
+bounded size, scaled to the benchmark size by P log2 P.
+
+ensemble: the serving mix (Poisson arrivals at 20/s of mixed runs), its
+throughput and p50/p99 completion latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 512]
 """
@@ -60,7 +67,9 @@ def pass_bytes(N, prec, les=False):
 
 
 def model_bytes(N, prec, les=False):
-    """BASELINE.md's model: n_fft * 2 * s_t * M^3 + 12 * 16 * K_N."""
+    """Compulsory-byte model of a step: n_fft * 2 * s_t * M^3 + 12 * 16 * K_N
+    (one write + read of an M^3 field per 3-D transform, the FP64 state and
+    history read and written once)."""
     M, nk1, nk2, nk3 = dims(N)
     n_fft = 21 if les else 15
     s_t = 8 if prec == 0 else 4
@@ -77,9 +86,14 @@ def sync_all(solver):
         pass
 
 
+def workload_nu(N):
+    """nu of the forced-DNS workloads: 2e-4 at 512^3, 8e-5 at 1024^3."""
+    return 8e-5 if N >= 1024 else 2e-4
+
+
 def run_form(N, prec, steps, warmup, seed, prof_steps):
     import spns_amd
-    s = spns_amd.Solver(Nx=N, nu=2e-4, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2,
+    s = spns_amd.Solver(Nx=N, nu=workload_nu(N), C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2,
                         precision=prec, use_graph=True)
     s.init_random(seed=seed, k0=4.0, urms=1.0)
     s.finish_init(False)
@@ -168,6 +182,26 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4, "launches": 3}))
 
 
+def copy_peak(mib=4096, reps=5):
+    """Measured device-to-device copy bandwidth (torch copy of mib MiB, HIP
+    events): the practical HBM ceiling beside the 8 TB/s data-sheet peak."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"GB_s": 2.0 * n * 4 / (ms * 1e-3) / 1e9, "bytes_per_copy": 2 * n * 4, "ms": ms,
+            "how": "torch device-to-device copy of %d MiB, mean of %d, read+write bytes" % (mib, reps)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,16 +216,32 @@ def main():
                     help="requests of the serving mix to run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--copy-peak", action="store_true", help="only print the measured copy bandwidth and exit")
+    ap.add_argument("--no-copy-peak", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
+    if a.copy_peak:
+        print(json.dumps(copy_peak()))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs one GPU; the step shards as a slab decomposition only in design "
-              "(see DESIGN.md).  Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: several GPUs would shard the step as a slab decomposition "
+              "(one all-to-all per 3-D transform), which is not implemented.  Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    copy = None
+    if not a.no_copy_peak:
+        # in a child process: torch's HIP runtime and the library's stay apart
+        import subprocess
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-peak"], capture_output=True, text=True)
+        if r.returncode == 0 and r.stdout.strip():
+            copy = json.loads(r.stdout.strip().splitlines()[-1])
+            print("bench: device copy %.0f GB/s" % copy["GB_s"], file=sys.stderr, flush=True)
+        else:
+            print("bench: copy peak not measured: %s" % r.stderr.strip()[-300:], file=sys.stderr, flush=True)
 
     N = a.N
     M = dims(N)[0]
@@ -285,14 +335,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32/f64 split" if main_form == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": "forced DNS 512^3 retained modes (FFT grid %d^3), nu=2e-4, constant power 0.1 "
-                               "on 1<=K<=2, CFL C_At=0.5, k0=4 u_rms=1 seed %d" % (M, a.seed),
+        "config": {"workload": "forced DNS %d^3 retained modes (FFT grid %d^3), nu=%g, constant power 0.1 "
+                               "on 1<=K<=2, CFL C_At=0.5, k0=4 u_rms=1 seed %d" % (N, M, workload_nu(N), a.seed),
                    "N": N, "M": M, "form": main_form, "parallelism": "1 GPU"},
         "forms": forms_out,
         "fp32_split_speedup": (forms_out["fp32"]["steps_per_s"] / forms_out["fp64"]["steps_per_s"]
                                if "fp32" in forms_out and "fp64" in forms_out else None),
         "fp32_split_error": split_err,
         "roofline": roof,
+        "copy_peak": copy,
         "cpu_baseline": cpu,
         "ensemble": ens,
     }
