@@ -224,16 +224,30 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
 // ---------------------------------------------------------------------------
 // Block reductions
 // ---------------------------------------------------------------------------
+// Largest power of two <= n.
+__host__ __device__ constexpr int pow2_floor(int n)
+{
+    int p = 1;
+    while (2 * p <= n) p *= 2;
+    return p;
+}
+
+// Fixed-order block reductions for any NT: the NT - P top values (P the
+// largest power of two <= NT) are folded onto the first ones, then a tree.
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* sh)
 {
-    // fixed-order tree: deterministic for a given NT
+    constexpr int P = pow2_floor(NT);
     const int t = threadIdx.x;
     __syncthreads();
     sh[t] = v;
     __syncthreads();
+    if constexpr (P < NT) {
+        if (t < NT - P) sh[t] = sh[t] + sh[t + P];
+        __syncthreads();
+    }
 #pragma unroll
-    for (int s = NT / 2; s > 0; s >>= 1) {
+    for (int s = P / 2; s > 0; s >>= 1) {
         if (t < s) sh[t] = sh[t] + sh[t + s];
         __syncthreads();
     }
@@ -245,12 +259,17 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* sh)
 {
+    constexpr int P = pow2_floor(NT);
     const int t = threadIdx.x;
     __syncthreads();
     sh[t] = v;
     __syncthreads();
+    if constexpr (P < NT) {
+        if (t < NT - P) sh[t] = fmax(sh[t], sh[t + P]);
+        __syncthreads();
+    }
 #pragma unroll
-    for (int s = NT / 2; s > 0; s >>= 1) {
+    for (int s = P / 2; s > 0; s >>= 1) {
         if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
         __syncthreads();
     }

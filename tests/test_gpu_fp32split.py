@@ -1,7 +1,7 @@
 """FP32-split form (FP32 transforms and products, FP64 state, history,
 reductions, forcing normalisation and time step) against the FP64 oracle.
 
-Stated tolerances (north-star target: E(k) within 1e-5 of FP64):
+Stated tolerances (the target is E(k) within 1e-5 of FP64):
 E(k) rel L2 <= 1e-5, epsilon rel <= 1e-5, skewness abs <= 1e-4,
 time and dt rel <= 1e-6, state rel L2 <= 1e-4."""
 import numpy as np
@@ -20,6 +20,11 @@ def rel(a, b):
     dict(Nx=64, nu=2e-3, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2),
     dict(Nx=32, nu=1e-4, C_Smag=0.17, C_At=0.5),
     dict(Nx=64, nu=1e-3, C_At=0.5, omega=(0.0, 0.0, 5.0)),
+    # z transforms of the 512^3 and 1024^3 runs (768 and 1536 points, 257 and
+    # 513 retained k3) on a small x-y cross-section
+    dict(Nx=16, Ny=16, Nz=512, nu=2e-3, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2),
+    dict(Nx=16, Ny=16, Nz=1024, nu=2e-3, C_At=0.5),
+    dict(Nx=16, Mz=768, nu=2e-3, C_At=0.5, C_Smag=0.17),
 ])
 def test_fp32_split_errors(case):
     import spns_amd

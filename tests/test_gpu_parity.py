@@ -63,10 +63,15 @@ CASES = {
     "self_adaptive": dict(Nx=32, nu=1e-3, C_At=0.5, isSelfAdaptive=True),
     "rect_grid": dict(Nx=64, Ny=32, Nz=48, nu=2e-3, C_At=0.5),
     "les_rot_64": dict(Nx=64, nu=5e-4, C_Smag=0.16, C_At=0.5, omega=(0.0, 0.0, 5.0)),
-    # oversampled z axis: exercises the wave-per-pencil z pass (lengths 512, 768, 1536)
+    # oversampled z axis: z transforms of lengths 512, 768, 1536 with few retained k3
     "zwave512": dict(Nx=32, Mz=512, nu=1e-3, C_At=0.5),
     "zwave768_forced": dict(Nx=32, Mz=768, nu=2e-3, C_At=0.5, force_eps=0.1),
     "zwave1536_sa": dict(Nx=32, Mz=1536, nu=1e-3, C_At=0.5, isSelfAdaptive=True),
+    # the z transform of the 512^3 run (768 points, 257 retained k3) on a
+    # small x-y cross-section, DNS and LES
+    "z768_full_k3": dict(Nx=16, Ny=16, Nz=512, nu=2e-3, C_At=0.5, force_eps=0.1),
+    "z768_full_k3_les": dict(Nx=16, Ny=16, Nz=512, nu=1e-4, C_Smag=0.17, C_At=0.5),
+    "z1536_full_k3": dict(Nx=16, Ny=16, Nz=1024, nu=2e-3, C_At=0.5),
 }
 
 
