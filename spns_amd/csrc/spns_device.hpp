@@ -396,6 +396,7 @@ struct FftS {
             const int idx = tid + q * NT;
             if (NBF % NT == 0 || idx < NBF) {
                 const int b = idx % NB, j = idx / NB;
+                __builtin_assume(j >= 0 && j < LR);  // idx < NBF = LR * NB
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     if constexpr (S == 0) v[q][r] = load(b, j + r * LR);
@@ -409,6 +410,7 @@ struct FftS {
             const int idx = tid + q * NT;
             if (NBF % NT == 0 || idx < NBF) {
                 const int b = idx % NB, j = idx / NB;
+                __builtin_assume(j >= 0 && j < LR);  // idx < NBF = LR * NB
                 const int k = j % Ns;
                 if constexpr (S > 0) {
                     if constexpr (sizeof(T) == 8) {

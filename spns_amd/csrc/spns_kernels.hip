@@ -507,7 +507,11 @@ __host__ __device__ constexpr bool zstage_fits()
 // kernel from 184 to 64-80 VGPRs and two blocks per CU (9.2-9.9 ms against
 // 13.7 ms at 512^3); in FP64 the LDS allows one block per CU anyway and the
 // recomputation only costs time, so it is off there.
-template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int MINW = 1, bool ZOPAQUE = (sizeof(T) == 4)>
+// NKC > 0: the retained k3 count is the compile-time constant NKC (the
+// standard 3/2 padding, nk3 = L/3 + 1), so the pruning of the first inverse
+// stage resolves at compile time and its zero inputs fold away.
+template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int MINW = 1, bool ZOPAQUE = (sizeof(T) == 4),
+          int NKC = 0>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void k_zstage(ZArgs a)
 {
     static_assert(LD >= (MODE == 0 ? 6 : 2), "pair buffer pitch too small");
@@ -568,7 +572,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
         constexpr int NPI = (MODE == 0) ? 6 : 2;
         const int tl = ZOPAQUE ? opaque(t) : t;
         auto load_inv = [&](int b, int p) -> cx<T> {
-            const bool lo = p < nk, hi = p > L - nk;
+            const int nkv = NKC > 0 ? NKC : nk;
+            const bool lo = p < nkv, hi = p > L - nkv;
             if (!(lo || hi)) return cx<T>{(T)0, (T)0};
             const int k = lo ? p : L - p;
             const T kk = (T)k;
@@ -1351,6 +1356,14 @@ hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, c
 #endif
 
 #if SPNS_PART == 0 || SPNS_PART == 3
+// Compile-time retained k3 count of the standard 3/2 padding (M = 3N/2,
+// nk3 = N/2 + 1 = M/3 + 1), 0 where M is not a multiple of 3.
+template <int L>
+constexpr int zstage_nkc()
+{
+    return (L % 3 == 0) ? L / 3 + 1 : 0;
+}
+
 // Threads of the staged z pass.  FP32 at L = 768: 12 waves (measured 9.5 ms
 // against 9.9 ms with 8 waves at 512^3); otherwise as the block kernel.
 template <typename T, int L>
@@ -1379,6 +1392,13 @@ static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
     constexpr int NTS = zstage_nt<T, L>();
     if constexpr (zstage_fits<T, L, NTS, MODE>()) {
         if (a.kernel == ZK_STAGED && zstage_ok<T, L, MODE>(a)) {
+            if constexpr (zstage_nkc<L>() > 0) {
+                if (a.nk3 == zstage_nkc<L>()) {
+                    hipLaunchKernelGGL((k_zstage<T, L, NTS, MODE, zstage_ld<MODE>(), 1, sizeof(T) == 4, zstage_nkc<L>()>),
+                                       dim3(grid), dim3(NTS), 0, st, a);
+                    return hipGetLastError();
+                }
+            }
             hipLaunchKernelGGL((k_zstage<T, L, NTS, MODE>), dim3(grid), dim3(NTS), 0, st, a);
             return hipGetLastError();
         }
@@ -1399,7 +1419,17 @@ static int zpass_grid_L(const ZArgs& a, int64_t npen)
             int dev = 0, ncu = 0, nb = 0;
             if (hipGetDevice(&dev) != hipSuccess) return -1;
             if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zstage<T, L, NT, 0>, NT, 0) != hipSuccess) return -1;
+            hipError_t e;
+            if constexpr (zstage_nkc<L>() > 0) {
+                if (a.nk3 == zstage_nkc<L>())
+                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &nb, k_zstage<T, L, NT, 0, zstage_ld<0>(), 1, sizeof(T) == 4, zstage_nkc<L>()>, NT, 0);
+                else
+                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zstage<T, L, NT, 0>, NT, 0);
+            } else {
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zstage<T, L, NT, 0>, NT, 0);
+            }
+            if (e != hipSuccess) return -1;
             const int64_t g = (int64_t)ncu * (nb > 0 ? nb : 1);
             return (int)(npen < g ? npen : g);
         }
