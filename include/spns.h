@@ -58,7 +58,8 @@ typedef struct spns_config {
     int32_t force_kmin, force_kmax;
     int32_t device;          /* HIP device ordinal */
     int32_t use_graph;       /* capture the step as a hipGraph */
-    int32_t zpass_variant;   /* z-pass kernel: 0 default (staged inputs), 1 block per pencil, 2 wave per pencil */
+    int32_t zpass_variant;   /* z-pass kernel: 0 default (staged inputs; DNS forward folded into the next
+                                pencil's inverse), 1 block per pencil, 2 wave per pencil, 3 staged, not folded */
     int32_t reserved[5];
 } spns_config;
 

@@ -173,7 +173,7 @@ struct spns_ctx {
     double* forced = nullptr;
     double* ft = nullptr;
     int grid_epi = 0, grid_z = 0;
-    int zkernel = ZK_STAGED;  // z-pass kernel (environment SPNS_ZPASS=staged|block|wave overrides)
+    int zkernel = ZK_STAGED;  // z-pass kernel (environment SPNS_ZPASS=staged|nofuse|block|wave overrides)
     hipGraphExec_t graph = nullptr;
     bool graph_ready = false;
     std::string err;
@@ -226,10 +226,13 @@ struct spns_ctx {
         sa = cfg.isSelfAdaptive != 0;
         nfwd = les ? 9 : 3;
 
-        zkernel = cfg.zpass_variant == 1 ? ZK_BLOCK : (cfg.zpass_variant == 2 ? ZK_WAVE : ZK_STAGED);
+        zkernel = cfg.zpass_variant == 1 ? ZK_BLOCK
+                  : cfg.zpass_variant == 2 ? ZK_WAVE
+                  : cfg.zpass_variant == 3 ? ZK_STAGED_NOFUSE
+                                           : ZK_STAGED;
         if (const char* zk = std::getenv("SPNS_ZPASS")) {
             const std::string v(zk);
-            zkernel = v == "wave" ? ZK_WAVE : (v == "block" ? ZK_BLOCK : ZK_STAGED);
+            zkernel = v == "wave" ? ZK_WAVE : v == "block" ? ZK_BLOCK : v == "nofuse" ? ZK_STAGED_NOFUSE : ZK_STAGED;
         }
         HIPCHK(hipSetDevice(cfg.device));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));

@@ -374,7 +374,37 @@ __device__ __forceinline__ uint32_t lds_addr(P* p)
 //   * hook() runs once, right after the barrier that ends stage 0, when
 //     every wave has finished reading stage 0's inputs.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false>
+// Number of leading entries of the forward twiddle table that FftS with
+// twiddle mode TWM (below) reads for length L.
+template <int L, int TWM>
+__host__ __device__ constexpr int fft_twiddle_span()
+{
+    int m = 1;
+    for (int S = 1; S < Plan<L>::n; S++) {
+        const int R = Plan<L>::r(S);
+        int Ns = 1;
+        for (int s = 0; s < S; s++) Ns *= Plan<L>::r(s);
+        const int TWS = L / (Ns * R);
+        int rmax = R - 1;
+        if (TWM == 1) rmax = 1;
+        if (TWM == 2) {
+            rmax = 1;
+            while (2 * rmax <= R - 1) rmax *= 2;
+        }
+        const int n = rmax * (Ns - 1) * TWS + 1;
+        m = n > m ? n : m;
+    }
+    return m;
+}
+
+// TWM, the twiddle source of a butterfly's R-1 factors W^r:
+//   0: R-1 reads of the table;
+//   1: one read (W) and W^r by repeated multiplication (R-2 products, error
+//      up to R-2 roundings);
+//   2: reads of the power-of-two exponents and one product for each other r
+//      (W^r = W^h W^(r-h), h the highest power of two below r; at most
+//      popcount(r)-1 roundings).
+template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false, int TWM = (sizeof(T) == 8 ? 1 : 2)>
 struct FftS {
     using C = cx<T>;
     static constexpr int NS = Plan<L>::n;
@@ -413,7 +443,7 @@ struct FftS {
                 __builtin_assume(j >= 0 && j < LR);  // idx < NBF = LR * NB
                 const int k = j % Ns;
                 if constexpr (S > 0) {
-                    if constexpr (sizeof(T) == 8) {
+                    if constexpr (TWM == 1) {
                         C w1 = twl[k * TWS];
                         if (DIR > 0) w1.y = -w1.y;
                         C w = w1;
@@ -421,6 +451,20 @@ struct FftS {
                         for (int r = 1; r < R; r++) {
                             v[q][r] = cmul(v[q][r], w);
                             if (r + 1 < R) w = cmul(w, w1);
+                        }
+                    } else if constexpr (TWM == 2) {
+                        C w[R];
+#pragma unroll
+                        for (int r = 1; r < R; r++) {
+                            if ((r & (r - 1)) == 0) {
+                                w[r] = twl[r * k * TWS];
+                                if (DIR > 0) w[r].y = -w[r].y;
+                            } else {
+                                int h = 1;
+                                while (2 * h < r) h *= 2;
+                                w[r] = cmul(w[h], w[r - h]);
+                            }
+                            v[q][r] = cmul(v[q][r], w[r]);
                         }
                     } else {
 #pragma unroll

@@ -41,14 +41,14 @@ struct StridedArgs {
     const void* tw;
 };
 
-enum { ZK_WAVE = 0, ZK_BLOCK = 1, ZK_STAGED = 2 };
+enum { ZK_WAVE = 0, ZK_BLOCK = 1, ZK_STAGED = 2, ZK_STAGED_NOFUSE = 3 };
 
 struct ZArgs {
     void* f[9];  // X2 fields: u v w | ux vx wx | uy vy wy ; pencil offset pen*pitch
     int nk3, pitch;
     int les;
     int sa;  // also reduce the self-adaptive sum form of the CFL velocity
-    int kernel;  // ZK_WAVE, ZK_BLOCK or ZK_STAGED
+    int kernel;  // ZK_WAVE, ZK_BLOCK, ZK_STAGED (DNS fused where it fits) or ZK_STAGED_NOFUSE
     int64_t npencil;
     double Ax, Ay, Az;  // CFL divisors (src/HIT_LES.h:56-58)
     double ct;          // (C_Smag*delta)^2 (src/HIT_LES.cpp:132)

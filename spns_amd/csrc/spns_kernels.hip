@@ -511,7 +511,7 @@ __host__ __device__ constexpr bool zstage_fits()
 // standard 3/2 padding, nk3 = L/3 + 1), so the pruning of the first inverse
 // stage resolves at compile time and its zero inputs fold away.
 template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int MINW = 1, bool ZOPAQUE = (sizeof(T) == 4),
-          int NKC = 0>
+          int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2)>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void k_zstage(ZArgs a)
 {
     static_assert(LD >= (MODE == 0 ? 6 : 2), "pair buffer pitch too small");
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
             if (nxt < a.npencil) issue(nxt);
         };
         auto none = [&]() {};
-        FftS<T, L, NPI, NT, LD, +1>::template run<false>(fbuf, twl, load_inv, store_lds, prefetch, tl);
+        FftS<T, L, NPI, NT, LD, +1, false, TWM>::template run<false>(fbuf, twl, load_inv, store_lds, prefetch, tl);
         lds_barrier();
 
         // -------- physical space
@@ -647,8 +647,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
         }
         if constexpr (MODE == 0) {
             auto load_lds = [&](int b, int p) -> cx<T> { return fbuf[p * LD + b]; };
-            if (a.les) FftS<T, L, 5, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
-            else FftS<T, L, 2, NT, LD, -1, true>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
+            if (a.les) FftS<T, L, 5, NT, LD, -1, true, TWM>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
+            else FftS<T, L, 2, NT, LD, -1, true, TWM>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
             lds_barrier();
             dma_wait();  // the next pencil's sources (this wave's copies) have landed
             const int npairs = a.les ? 5 : 2;
@@ -684,6 +684,163 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
             a.skew_partial[2 * blockIdx.x] = s2;
             a.skew_partial[2 * blockIdx.x + 1] = s3;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DNS z pass with the forward transforms of pencil n-1 folded into the
+// inverse transforms of pencil n (the default DNS form).
+// One Stockham transform of 8 columns per pencil step:
+//   columns 0-5: the six paired c2r inputs of pencil n (from the staging
+//                area, as in k_zstage);
+//   columns 6-7: the products (cx, cy) and (cz, 0) of pencil n-1, written
+//                by the previous step's physical-space loop.
+// All columns run as inverse transforms; the forward ones through
+// conj(inverse(conj x)), so one set of twiddles and butterflies serves both
+// and every stage keeps all threads busy (768 = 64 x 12: 512 then 768
+// butterflies at 768 threads).  Per step: the transform (the staging copy
+// of pencil n+1 is issued after its first stage), then the unpack of
+// pencil n-1's half spectra, then pencil n's products into columns 6-7.
+// The loop runs one step past the block's last pencil to finish it.
+// LDS row pitch 9 (odd), twiddle table trimmed to the entries read.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int NT, int NKC, int TWM = (sizeof(T) == 8 ? 1 : 2)>
+__host__ __device__ constexpr size_t zfuse_lds_bytes()
+{
+    return sizeof(cx<T>) * ((size_t)L * 9 + (size_t)9 * zstage_spc<L>() + fft_twiddle_span<L, TWM>()) +
+           3 * 9 * sizeof(void*);
+}
+
+template <typename T, int L, int NT, int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2), bool ZOPAQUE = (sizeof(T) == 4)>
+__global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
+{
+    constexpr int LD = 9, NB = 8;
+    constexpr int SPC = zstage_spc<L>();
+    constexpr int NW = NT / 64;
+    constexpr int EPS = 16 / (int)sizeof(cx<T>);
+    constexpr int TWN = fft_twiddle_span<L, TWM>();
+    static_assert(L * LD * sizeof(cx<T>) >= NT * sizeof(double), "reduction scratch aliases the pair buffer");
+    __shared__ __align__(16) cx<T> fbuf[L * LD];
+    __shared__ __align__(16) cx<T> stg[9 * SPC];
+    __shared__ __align__(16) cx<T> twl[TWN];
+    __shared__ const cx<T>* srcp[9];
+    __shared__ cx<T>* dstp[3];
+    const int t = threadIdx.x;
+    const int nk = a.nk3, pitch = a.pitch;
+    const int SP = pitch + zstage_pad<T>();
+    const int nslot = 9 * SP / EPS;
+    T mu = (T)0, mv = (T)0, mw = (T)0;
+    double vmax_sa = 0.0, s2 = 0.0, s3 = 0.0;
+    const double iAx = a.Ax, iAy = a.Ay, iAz = a.Az;
+    const bool sa = a.sa;
+
+    if (t < 9) srcp[t] = (const cx<T>*)a.f[t];
+    if (t < 3) dstp[t] = (cx<T>*)a.f[t];
+    for (int i = t; i < TWN; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
+    __syncthreads();
+
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+    const uint32_t stg_base = lds_addr(stg);
+    auto issue = [&](int64_t pen) {
+        const int64_t off = pen * (int64_t)pitch;
+        for (int m = wave; m * 64 < nslot; m += NW) {
+            const int s = m * 64 + lane;
+            const int e = s * EPS;
+            const int f = e / SP, k = e - f * SP;
+            if (s < nslot && k < pitch) dma16(srcp[f] + off + k, stg_base + (uint32_t)m * 1024u);
+        }
+    };
+    // inverse pair b (0..5): logical fields (2b, 2b+1) of u v w ux uy uz vx vy vz wx wy wz
+    constexpr uint32_t SRC_A = 0x814620u, SRC_B = 0x257031u, DZ_A = 0x10u, DZ_B = 0x24u;
+
+    int64_t pen = blockIdx.x;
+    if (pen < a.npencil) issue(pen);
+    dma_wait();
+    lds_barrier();
+    int64_t prev = -1;  // pencil whose products sit in columns 6-7
+    for (;; pen += gridDim.x) {
+        const bool have = pen < a.npencil;
+        const bool havep = prev >= 0;
+        if (!have && !havep) break;
+        const int tl = ZOPAQUE ? opaque(t) : t;
+        const int64_t nxt = pen + gridDim.x;
+        auto load = [&](int b, int p) -> cx<T> {
+            if (b >= 6) {
+                if (!havep) return cx<T>{(T)0, (T)0};
+                const cx<T> v = fbuf[p * LD + b];
+                return cx<T>{v.x, -v.y};  // forward = conj(inverse(conj x))
+            }
+            if (!have) return cx<T>{(T)0, (T)0};
+            const int nkv = NKC > 0 ? NKC : nk;
+            const bool lo = p < nkv, hi = p > L - nkv;
+            if (!(lo || hi)) return cx<T>{(T)0, (T)0};
+            const int k = lo ? p : L - p;
+            const T kk = (T)k;
+            const int fa = (SRC_A >> (4 * b)) & 15, fb = (SRC_B >> (4 * b)) & 15;
+            cx<T> A = stg[fa * SP + k];
+            if ((DZ_A >> b) & 1) A = cx<T>{-kk * A.y, kk * A.x};
+            cx<T> B = stg[fb * SP + k];
+            if ((DZ_B >> b) & 1) B = cx<T>{-kk * B.y, kk * B.x};
+            const T s = lo ? (T)1 : (T)-1;
+            const T ay = (p == 0) ? (T)0 : s * A.y;
+            const T by = (p == 0) ? (T)0 : s * B.y;
+            return cx<T>{A.x - by, ay + B.x};
+        };
+        auto store = [&](int b, int p, cx<T> v) { fbuf[p * LD + b] = (b >= 6) ? cx<T>{v.x, -v.y} : v; };
+        auto prefetch = [&]() {
+            if (have && nxt < a.npencil) issue(nxt);
+        };
+        FftS<T, L, NB, NT, LD, +1, true, TWM>::template run<false>(fbuf, twl, load, store, prefetch, tl);
+        lds_barrier();
+        dma_wait();  // pencil n+1's sources (this wave's copies) have landed
+        if (havep) {
+            // -------- unpack pencil n-1: (cx, cy) and (cz, -) half spectra
+            const int64_t offp = prev * (int64_t)pitch;
+            for (int idx = t; idx < 2 * nk; idx += NT) {
+                const int q = idx >= nk ? 1 : 0, k = idx - q * nk;
+                const cx<T> Z = fbuf[k * LD + 6 + q];
+                const cx<T> Zc = fbuf[((L - k) % L) * LD + 6 + q];
+                const T h = (T)0.5;
+                dstp[2 * q][offp + k] = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
+                if (q == 0) dstp[1][offp + k] = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
+            }
+            lds_barrier();
+        }
+        if (have) {
+            // -------- physical space of pencil n: products into columns 6-7
+            for (int n = t; n < L; n += NT) {
+                cx<T>* row = &fbuf[n * LD];
+                const T u = row[0].x, v = row[0].y, w = row[1].x;
+                mu = fmax(mu, fabs(u));
+                mv = fmax(mv, fabs(v));
+                mw = fmax(mw, fabs(w));
+                if (sa) vmax_sa = fmax(vmax_sa, fabs((double)u) / iAx + fabs((double)v) / iAy + fabs((double)w) / iAz);  // src/HIT_LES.cpp:377
+                const T ux = row[1].y, uy = row[2].x, uz = row[2].y;
+                const T vx = row[3].x, vy = row[3].y, vz = row[4].x;
+                const T wx = row[4].y, wy = row[5].x, wz = row[5].y;
+                s2 += (double)ux * (double)ux;
+                s3 += (double)ux * (double)ux * (double)ux;
+                // advective form (src/HIT_LES.cpp:118-120)
+                row[6] = cx<T>{u * ux + v * uy + w * uz, u * vx + v * vy + w * vz};
+                row[7] = cx<T>{u * wx + v * wy + w * wz, (T)0};
+            }
+            lds_barrier();
+        }
+        prev = have ? pen : -1;
+    }
+    double* red = (double*)fbuf;
+    const double bu = block_max<NT>((double)mu, red);
+    const double bv = block_max<NT>((double)mv, red);
+    const double bw = block_max<NT>((double)mw, red);
+    const double vmax = fmax(bu / iAx, fmax(bv / iAy, bw / iAz));  // src/HIT_LES.cpp:393
+    if (sa) vmax_sa = block_max<NT>(vmax_sa, red);
+    s2 = block_sum<NT>(s2, red);
+    s3 = block_sum<NT>(s3, red);
+    if (t == 0) {
+        atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
+        atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
+        a.skew_partial[2 * blockIdx.x] = s2;
+        a.skew_partial[2 * blockIdx.x + 1] = s3;
     }
 }
 
@@ -1380,61 +1537,117 @@ static bool zstage_ok(const ZArgs& a)
     return a.pitch + zstage_pad<T>() <= zstage_spc<L>() && a.nk3 <= a.pitch;
 }
 
+// z-pass kernel variants and the choice among them
+enum { ZV_BLOCK, ZV_WAVE, ZV_STAGE, ZV_STAGE_NKC, ZV_FUSE_NKC };
+
+template <typename T, int L>
+constexpr int zfuse_nt()
+{
+    return sstage_nt<L, 8>();
+}
+
+template <typename T, int L>
+constexpr bool zfuse_fits()
+{
+    return zstage_nkc<L>() > 0 && zfuse_lds_bytes<T, L, zfuse_nt<T, L>(), zstage_nkc<L>()>() <= 160 * 1024 - 512;
+}
+
+template <typename T, int L, int MODE>
+static int zpass_pick(const ZArgs& a)
+{
+    if constexpr (has_wave_plan<L>::value) {
+        if ((!a.les || MODE == 1) && a.kernel == ZK_WAVE) return ZV_WAVE;
+    }
+    if (a.kernel == ZK_STAGED || a.kernel == ZK_STAGED_NOFUSE) {
+        const bool pitch_ok = a.pitch + zstage_pad<T>() <= zstage_spc<L>() && a.nk3 <= a.pitch;
+        const bool nkc = zstage_nkc<L>() > 0 && a.nk3 == zstage_nkc<L>();
+        if constexpr (MODE == 0 && zfuse_fits<T, L>()) {
+            if (a.kernel == ZK_STAGED && !a.les && pitch_ok && nkc) return ZV_FUSE_NKC;
+        }
+        if constexpr (zstage_fits<T, L, zstage_nt<T, L>(), MODE>()) {
+            if (pitch_ok) return nkc ? ZV_STAGE_NKC : ZV_STAGE;
+        }
+    }
+    return ZV_BLOCK;
+}
+
 template <typename T, int L, int MODE>
 static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
 {
-    if constexpr (has_wave_plan<L>::value) {
-        if ((!a.les || MODE == 1) && a.kernel == ZK_WAVE) {
+    constexpr int NTS = zstage_nt<T, L>();
+    constexpr int NKC = zstage_nkc<L>();
+    switch (zpass_pick<T, L, MODE>(a)) {
+    case ZV_WAVE:
+        if constexpr (has_wave_plan<L>::value) {
             hipLaunchKernelGGL((k_zwave<T, L, MODE>), dim3(grid), dim3(256), 0, st, a);
             return hipGetLastError();
         }
-    }
-    constexpr int NTS = zstage_nt<T, L>();
-    if constexpr (zstage_fits<T, L, NTS, MODE>()) {
-        if (a.kernel == ZK_STAGED && zstage_ok<T, L, MODE>(a)) {
-            if constexpr (zstage_nkc<L>() > 0) {
-                if (a.nk3 == zstage_nkc<L>()) {
-                    hipLaunchKernelGGL((k_zstage<T, L, NTS, MODE, zstage_ld<MODE>(), 1, sizeof(T) == 4, zstage_nkc<L>()>),
-                                       dim3(grid), dim3(NTS), 0, st, a);
-                    return hipGetLastError();
-                }
-            }
+        break;
+    case ZV_FUSE_NKC:
+        if constexpr (MODE == 0 && zfuse_fits<T, L>()) {
+            constexpr int NTF = zfuse_nt<T, L>();
+            hipLaunchKernelGGL((k_zfuse<T, L, NTF, NKC>), dim3(grid), dim3(NTF), 0, st, a);
+            return hipGetLastError();
+        }
+        break;
+    case ZV_STAGE_NKC:
+        if constexpr (NKC > 0 && zstage_fits<T, L, NTS, MODE>()) {
+            hipLaunchKernelGGL((k_zstage<T, L, NTS, MODE, zstage_ld<MODE>(), 1, sizeof(T) == 4, NKC>), dim3(grid), dim3(NTS),
+                               0, st, a);
+            return hipGetLastError();
+        }
+        break;
+    case ZV_STAGE:
+        if constexpr (zstage_fits<T, L, NTS, MODE>()) {
             hipLaunchKernelGGL((k_zstage<T, L, NTS, MODE>), dim3(grid), dim3(NTS), 0, st, a);
             return hipGetLastError();
         }
+        break;
+    default:
+        break;
     }
     constexpr int NT = zpass_nt<L>();
     hipLaunchKernelGGL((k_zpass<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
     return hipGetLastError();
 }
 
-// Grid of the z pass: the staged kernel runs persistent blocks (as many as
-// fit on the device at once), the others one block per pencil up to 65536.
+template <class K>
+static int blocks_per_cu(K kernel, int nt)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, nt, 0) != hipSuccess) return -1;
+    return nb > 0 ? nb : 1;
+}
+
+// Grid of the z pass: the staged kernels run persistent blocks (as many as
+// fit on the device at once, counted for the full step's kernel), the others
+// one block per pencil up to 65536.
 template <typename T, int L>
 static int zpass_grid_L(const ZArgs& a, int64_t npen)
 {
-    if (a.kernel == ZK_STAGED && zstage_ok<T, L, 0>(a)) {
-        constexpr int NT = zstage_nt<T, L>();
-        if constexpr (zstage_fits<T, L, NT, 0>()) {
-            int dev = 0, ncu = 0, nb = 0;
-            if (hipGetDevice(&dev) != hipSuccess) return -1;
-            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-            hipError_t e;
-            if constexpr (zstage_nkc<L>() > 0) {
-                if (a.nk3 == zstage_nkc<L>())
-                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &nb, k_zstage<T, L, NT, 0, zstage_ld<0>(), 1, sizeof(T) == 4, zstage_nkc<L>()>, NT, 0);
-                else
-                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zstage<T, L, NT, 0>, NT, 0);
-            } else {
-                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zstage<T, L, NT, 0>, NT, 0);
-            }
-            if (e != hipSuccess) return -1;
-            const int64_t g = (int64_t)ncu * (nb > 0 ? nb : 1);
-            return (int)(npen < g ? npen : g);
-        }
+    constexpr int NTS = zstage_nt<T, L>();
+    constexpr int NKC = zstage_nkc<L>();
+    int nb = 0;
+    switch (zpass_pick<T, L, 0>(a)) {
+    case ZV_FUSE_NKC:
+        if constexpr (zfuse_fits<T, L>()) nb = blocks_per_cu(k_zfuse<T, L, zfuse_nt<T, L>(), NKC>, zfuse_nt<T, L>());
+        break;
+    case ZV_STAGE_NKC:
+        if constexpr (NKC > 0 && zstage_fits<T, L, NTS, 0>())
+            nb = blocks_per_cu(k_zstage<T, L, NTS, 0, zstage_ld<0>(), 1, sizeof(T) == 4, NKC>, NTS);
+        break;
+    case ZV_STAGE:
+        if constexpr (zstage_fits<T, L, NTS, 0>()) nb = blocks_per_cu(k_zstage<T, L, NTS, 0>, NTS);
+        break;
+    default:
+        return (int)(npen < 65536 ? npen : 65536);
     }
-    return (int)(npen < 65536 ? npen : 65536);
+    if (nb <= 0) return -1;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    const int64_t g = (int64_t)ncu * nb;
+    return (int)(npen < g ? npen : g);
 }
 
 int zpass_grid(int L, int prec, const ZArgs& a, int64_t npen)
@@ -1446,14 +1659,16 @@ int zpass_grid(int L, int prec, const ZArgs& a, int64_t npen)
     return -1;
 }
 
-// Kernel a z pass with these arguments runs: ZK_STAGED falls back to
-// ZK_BLOCK where the staged kernel's LDS does not fit.
+// Kernel a z pass with these arguments runs (ZK_*; the staged forms report
+// ZK_STAGED): falls back to ZK_BLOCK where the staged kernels' LDS does not fit.
 template <typename T, int L>
 static int zpass_kernel_L(const ZArgs& a)
 {
-    if (a.kernel == ZK_WAVE && has_wave_plan<L>::value && !a.les) return ZK_WAVE;
-    if (a.kernel == ZK_STAGED && zstage_ok<T, L, 0>(a)) return ZK_STAGED;
-    return ZK_BLOCK;
+    switch (zpass_pick<T, L, 0>(a)) {
+    case ZV_WAVE: return ZK_WAVE;
+    case ZV_BLOCK: return ZK_BLOCK;
+    default: return ZK_STAGED;
+    }
 }
 
 int zpass_kernel(int L, int prec, const ZArgs& a)
