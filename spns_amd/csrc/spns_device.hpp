@@ -404,7 +404,12 @@ __host__ __device__ constexpr int fft_twiddle_span()
 //   2: reads of the power-of-two exponents and one product for each other r
 //      (W^r = W^h W^(r-h), h the highest power of two below r; at most
 //      popcount(r)-1 roundings).
-template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false, int TWM = (sizeof(T) == 8 ? 1 : 2)>
+// S0COL: stage 0 maps consecutive threads to consecutive rows of one column
+// (b = idx / (L/R0), j = idx % (L/R0)) instead of across columns, so with
+// L/R0 a multiple of 64 every wave of stage 0 works on a single column and a
+// column-dependent loader runs without divergence.
+template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false, int TWM = (sizeof(T) == 8 ? 1 : 2),
+          bool S0COL = false>
 struct FftS {
     using C = cx<T>;
     static constexpr int NS = Plan<L>::n;
@@ -425,8 +430,10 @@ struct FftS {
         for (int q = 0; q < BPT; q++) {
             const int idx = tid + q * NT;
             if (NBF % NT == 0 || idx < NBF) {
-                const int b = idx % NB, j = idx / NB;
+                const int b = (S == 0 && S0COL) ? idx / LR : idx % NB;
+                const int j = (S == 0 && S0COL) ? idx % LR : idx / NB;
                 __builtin_assume(j >= 0 && j < LR);  // idx < NBF = LR * NB
+                __builtin_assume(b >= 0 && b < NB);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     if constexpr (S == 0) v[q][r] = load(b, j + r * LR);
@@ -439,8 +446,10 @@ struct FftS {
         for (int q = 0; q < BPT; q++) {
             const int idx = tid + q * NT;
             if (NBF % NT == 0 || idx < NBF) {
-                const int b = idx % NB, j = idx / NB;
+                const int b = (S == 0 && S0COL) ? idx / LR : idx % NB;
+                const int j = (S == 0 && S0COL) ? idx % LR : idx / NB;
                 __builtin_assume(j >= 0 && j < LR);  // idx < NBF = LR * NB
+                __builtin_assume(b >= 0 && b < NB);
                 const int k = j % Ns;
                 if constexpr (S > 0) {
                     if constexpr (TWM == 1) {
