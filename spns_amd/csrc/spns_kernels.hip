@@ -132,7 +132,8 @@ __host__ __device__ constexpr int sstage_nt()
 template <typename TIN, typename T, int L, int TB, int DIR>
 __host__ __device__ constexpr size_t sstage_lds_bytes()
 {
-    return sizeof(cx<T>) * (size_t)L * (TB + 1) + sizeof(cx<TIN>) * (size_t)sstage_rows<L, DIR>() * TB;
+    return sizeof(cx<T>) * ((size_t)L * TB + fft_twiddle_span<L, sizeof(T) == 8 ? 1 : 2>()) +
+           sizeof(cx<TIN>) * (size_t)sstage_rows<L, DIR>() * TB;
 }
 
 template <typename TIN, typename T, int L, int TB, int DIR>
@@ -148,9 +149,11 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
     constexpr int SPR = TB * (int)sizeof(cx<TIN>) / 16;  // 16-byte copy slots per staged row
     constexpr int EPS = 16 / (int)sizeof(cx<TIN>);        // input elements per slot
     constexpr int NW = NT / 64;
+    constexpr int TWM = sizeof(T) == 8 ? 1 : 2;
+    constexpr int TWN = fft_twiddle_span<L, TWM>();  // the table entries the transform reads
     __shared__ __align__(16) cx<T> fbuf[L * TB];
     __shared__ __align__(16) cx<TIN> stg[SROWS * TB];
-    __shared__ __align__(16) cx<T> twl[L];
+    __shared__ __align__(16) cx<T> twl[TWN];
     const int t = threadIdx.x;
     const int hin = a.nin / 2, hout = a.nout / 2;
     const int ntiles = (a.B + TB - 1) / TB;
@@ -159,7 +162,7 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
     const int G = gridDim.x;  // a multiple of 8
     const int64_t lin = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
 
-    for (int i = t; i < L; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
+    for (int i = t; i < TWN; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
     __syncthreads();
 
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
         auto prefetch = [&]() {
             if (nxt < nitems) issue(nxt);
         };
-        FftS<T, L, TB, NT, TB, DIR>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
+        FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
         lds_barrier();
         dma_wait();  // the next item's tile (this wave's copies) has landed
         cx<TOUT>* out = (cx<TOUT>*)a.job[jb].out + ai * a.out_a_stride + b0;
