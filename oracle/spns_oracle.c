@@ -474,7 +474,7 @@ static void real_loop_product(orc_t *o, double *out, const double *g0, const dou
 static void skew_accumulate(orc_t *o, const double *g)
 {
     const int Mx = o->p.Mx, My = o->p.My, Mz = o->p.Mz, nz = o->Mz2 + 1;
-    double s2 = 0.0, s3 = 0.0;
+    long double s2 = 0.0L, s3 = 0.0L; /* long double: see integrate_field */
 #pragma omp parallel for collapse(2) schedule(static) reduction(+ : s2, s3)
     for (int i = 0; i < Mx; i++)
         for (int j = 0; j < My; j++)
@@ -483,8 +483,8 @@ static void skew_accumulate(orc_t *o, const double *g)
                 s2 += x * x;
                 s3 += x * x * x;
             }
-    o->skew_sum2 = s2;
-    o->skew_sum3 = s3;
+    o->skew_sum2 = (double)s2;
+    o->skew_sum3 = (double)s3;
 }
 
 /* Convective term, advective form (src/HIT_LES.cpp:74-125).  For DNS the
@@ -562,13 +562,17 @@ static void sgs_divergence(orc_t *o)
 /* Shell integral (src/HIT_LES.cpp:1137-1214).  which: 0 kinetic energy
  * (src/HIT_LES.cpp:991-996), 1 enstrophy (:1000-1003).  If bins != NULL the
  * per-shell values are written for K in [Kmin,Kmax] and their sum returned;
- * otherwise the direct sum over modes is returned. */
+ * otherwise the direct sum over modes is returned.  Sums accumulate in long
+ * double: the oracle is the accuracy reference, and a plain sequential double
+ * sum over 10^7..10^8 modes drifts by up to 1e-12 relative on its own
+ * (summation order is not part of the reference's algorithm: its MPI
+ * reduction order changes with the rank count). */
 static double integrate_field(orc_t *o, int which, double *bins, int Kmin, int Kmax)
 {
     const int Mx = o->p.Mx, My = o->p.My, nz = o->Mz2 + 1;
     const double *u = o->uk[0], *v = o->uk[1], *w = o->uk[2];
     if (bins) {
-        for (int K = Kmin; K <= Kmax; K++) bins[K] = 0.0;
+        long double *lb = (long double *)calloc((size_t)(Kmax + 1), sizeof(long double));
         for (int b = 0; b < My; b++)
             for (int a = 0; a < Mx; a++)
                 for (int k3 = 0; k3 < nz; k3++) {
@@ -579,13 +583,17 @@ static double integrate_field(orc_t *o, int which, double *bins, int Kmin, int K
                     size_t r = 2 * ind, im = r + 1;
                     double ke = 0.5 * (u[r] * u[r] + u[im] * u[im] + v[r] * v[r] + v[im] * v[im] + w[r] * w[r] + w[im] * w[im]);
                     double aux = (which == 0) ? ke : 2 * o->rad2[ind] * ke;
-                    bins[K] += o->conjugate[ind] ? (2 * aux) : aux;
+                    lb[K] += o->conjugate[ind] ? (2 * aux) : aux;
                 }
-        double s = 0.0;
-        for (int K = Kmin; K <= Kmax; K++) s += bins[K];
-        return s;
+        long double s = 0.0L;
+        for (int K = Kmin; K <= Kmax; K++) {
+            bins[K] = (double)lb[K];
+            s += lb[K];
+        }
+        free(lb);
+        return (double)s;
     }
-    double s = 0.0;
+    long double s = 0.0L;
     for (int b = 0; b < My; b++)
         for (int a = 0; a < Mx; a++)
             for (int k3 = 0; k3 < nz; k3++) {
@@ -598,7 +606,7 @@ static double integrate_field(orc_t *o, int which, double *bins, int Kmin, int K
                 double aux = (which == 0) ? ke : 2 * o->rad2[ind] * ke;
                 s += o->conjugate[ind] ? (2 * aux) : aux;
             }
-    return s;
+    return (double)s;
 }
 
 double orc_energy(orc_t *o) /* src/HIT_LES.cpp:1254-1259 */
