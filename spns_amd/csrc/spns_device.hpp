@@ -87,38 +87,47 @@ __device__ __forceinline__ void dft8(cx<T>* x)
     x[3] = e3 + o3; x[7] = e3 - o3;
 }
 
+// 6 = 2 x 3 and 12 = 3 x 4 by the Good-Thomas prime-factor algorithm (the
+// factors are coprime, so the index maps n = (N2 n1 + N1 n2) mod N and
+// k = (N2 (N2^-1 mod N1) k1 + N1 (N1^-1 mod N2) k2) mod N leave no twiddles):
+//   6:  n = (3 n1 + 2 n2) mod 6,  k = (3 k1 + 4 k2) mod 6;
+//   12: n = (4 n1 + 3 n2) mod 12, k = (4 k1 + 9 k2) mod 12.
 template <int DIR, typename T>
 __device__ __forceinline__ void dft6(cx<T>* x)
 {
-    const T s = (T)0.86602540378443864676;
-    cx<T> e0 = x[0], e1 = x[2], e2 = x[4];
-    cx<T> o0 = x[1], o1 = x[3], o2 = x[5];
-    dft3<DIR>(e0, e1, e2);
-    dft3<DIR>(o0, o1, o2);
-    o1 = cmul(o1, wk<DIR>((T)0.5, s));
-    o2 = cmul(o2, wk<DIR>((T)-0.5, s));
-    x[0] = e0 + o0; x[3] = e0 - o0;
-    x[1] = e1 + o1; x[4] = e1 - o1;
-    x[2] = e2 + o2; x[5] = e2 - o2;
+    cx<T> y[2][3];
+#pragma unroll
+    for (int n1 = 0; n1 < 2; n1++) {
+#pragma unroll
+        for (int n2 = 0; n2 < 3; n2++) y[n1][n2] = x[(3 * n1 + 2 * n2) % 6];
+        dft3<DIR>(y[n1][0], y[n1][1], y[n1][2]);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 3; k2++) {
+        cx<T> a = y[0][k2], b = y[1][k2];
+        dft2<DIR>(a, b);
+        x[(4 * k2) % 6] = a;
+        x[(3 + 4 * k2) % 6] = b;
+    }
 }
 
 template <int DIR, typename T>
 __device__ __forceinline__ void dft12(cx<T>* x)
 {
-    const T s = (T)0.86602540378443864676;
-    cx<T> e[6] = {x[0], x[2], x[4], x[6], x[8], x[10]};
-    cx<T> o[6] = {x[1], x[3], x[5], x[7], x[9], x[11]};
-    dft6<DIR>(e);
-    dft6<DIR>(o);
-    o[1] = cmul(o[1], wk<DIR>(s, (T)0.5));
-    o[2] = cmul(o[2], wk<DIR>((T)0.5, s));
-    o[3] = mul_i<DIR>(o[3]);
-    o[4] = cmul(o[4], wk<DIR>((T)-0.5, s));
-    o[5] = cmul(o[5], wk<DIR>(-s, (T)0.5));
+    cx<T> y[3][4];
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        x[k] = e[k] + o[k];
-        x[k + 6] = e[k] - o[k];
+    for (int n1 = 0; n1 < 3; n1++) {
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) y[n1][n2] = x[(4 * n1 + 3 * n2) % 12];
+        dft4<DIR>(y[n1][0], y[n1][1], y[n1][2], y[n1][3]);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) {
+        cx<T> a0 = y[0][k2], a1 = y[1][k2], a2 = y[2][k2];
+        dft3<DIR>(a0, a1, a2);
+        x[(9 * k2) % 12] = a0;
+        x[(4 + 9 * k2) % 12] = a1;
+        x[(8 + 9 * k2) % 12] = a2;
     }
 }
 
