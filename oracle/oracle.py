@@ -65,6 +65,9 @@ def lib():
         L.orc_get_rhist.argtypes = [P, D]
         L.orc_velocity_physical.argtypes = [P, D]
         L.orc_parseval.argtypes = [P, D]
+        L.orc_invariants_point.argtypes = [D, D]
+        L.orc_keep_start.argtypes = [P]
+        L.orc_invariant_spectra.argtypes = [P, D]
         L.orc_get_scalars.argtypes = [P, D]
         L.orc_set_nu.argtypes = [P, ctypes.c_double]
         L.orc_last_rad_max.argtypes = [P]
@@ -199,6 +202,17 @@ class Oracle:
         self.L.orc_parseval(self.h, _dp(out))
         return out
 
+    def keep_start(self):
+        """Keep the state at the start of every step (for invariant_spectra)."""
+        self.L.orc_keep_start(self.h)
+
+    def invariant_spectra(self):
+        """Shell spectra of QG, QS, RG, RS, V2 of the velocity gradients at the
+        start of the last step: array [5][last_rad_max + 1]."""
+        out = np.zeros((5, self.last_rad_max + 1))
+        self.L.orc_invariant_spectra(self.h, _dp(out))
+        return out
+
     def velocity_physical(self):
         Mx, My, Mz = self.shape
         out = np.zeros((3, Mx, My, Mz))
@@ -209,6 +223,14 @@ class Oracle:
         s = np.zeros(len(SCALAR_NAMES))
         self.L.orc_get_scalars(self.h, _dp(s))
         return dict(zip(SCALAR_NAMES, s.tolist()))
+
+
+def invariants_point(g):
+    """QG, QS, RG, RS, V2 of one velocity-gradient tensor g[i][j] = d u_i / d x_j."""
+    g = np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(9))
+    out = np.zeros(5)
+    lib().orc_invariants_point(_dp(g), _dp(out))
+    return out
 
 
 def fft_c2r(cin, Mx, My, Mz):

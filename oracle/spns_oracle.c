@@ -306,6 +306,7 @@ typedef struct {
     fft3d fft;
     /* state */
     double *uk[3], *uk0[3], *ukaux[3], *R0[3], *R1[3], *conv_k[3];
+    double *uk_start[3]; /* state at the start of the last step (orc_keep_start) */
     double *u[3], *grad[9], *conv, *ctmp;
     double *nu_eddy, *sgs[6], *div_k[3];
     unsigned char *dealiased, *conjugate;
@@ -413,7 +414,7 @@ void orc_destroy(orc_t *o)
     if (!o) return;
     for (int c = 0; c < 3; c++) {
         free(o->uk[c]); free(o->R0[c]); free(o->R1[c]); free(o->conv_k[c]); free(o->u[c]);
-        free(o->uk0[c]); free(o->ukaux[c]); free(o->div_k[c]);
+        free(o->uk0[c]); free(o->ukaux[c]); free(o->div_k[c]); free(o->uk_start[c]);
     }
     for (int g = 0; g < 9; g++) free(o->grad[g]);
     free(o->conv); free(o->ctmp); free(o->nu_eddy);
@@ -829,6 +830,8 @@ void orc_recalc_timestep(orc_t *o)
 /* src/HIT_LES.cpp:9-25 */
 void orc_step(orc_t *o)
 {
+    if (o->uk_start[0])
+        for (int c = 0; c < 3; c++) memcpy(o->uk_start[c], o->uk[c], sizeof(double) * 2 * o->nc);
     r_vector(o);
     predictor(o);
     projection(o);
@@ -1136,6 +1139,127 @@ void orc_get_scalars(const orc_t *o, double *s)
 void orc_set_nu(orc_t *o, double nu) { o->p.nu = nu; }
 int orc_last_rad_max(const orc_t *o) { return o->last_rad_max; }
 int orc_last_rad(const orc_t *o) { return o->last_rad; }
+
+/* ------------------------------------------------------------------------ */
+/* Velocity-gradient invariant spectra (src/HIT_LES.cpp:1297-1319)           */
+/* Invariants of G[i][j] = d u_i / d x_j (src/tensor_algebra.cpp:250-303):
+ *   S = 0.5 G + 0.5 G^T (calc_force_symmetry), Omega = G - S (calc_Omega);
+ *   Q(T) = 0.5 ((tr T)^2 - tr(T T)) (calc_second_invariant), R(T) = det T
+ *   (calc_determinant, its term order);
+ *   QG = Q(G), QS = Q(S), RG = det G, RS = det S,
+ *   V2 = 4 (tr(S^2 Omega^2) - 2 QS QO), QO = QG - QS (calc_QO, :356).       */
+static double tr3(double A[3][3]) { double t = 0.0; for (int i = 0; i <= 2; i++) t += A[i][i]; return t; }
+
+static void mul3(double a[3][3], double b[3][3], double ab[3][3])
+{
+    for (int i = 0; i <= 2; i++)
+        for (int j = 0; j <= 2; j++) {
+            ab[i][j] = 0.0;
+            for (int k = 0; k <= 2; k++) ab[i][j] += a[i][k] * b[k][j];
+        }
+}
+
+static double second_inv(double T[3][3])
+{
+    double T2[3][3];
+    mul3(T, T, T2);
+    const double trT = tr3(T), trT2 = tr3(T2);
+    return 0.5 * (trT * trT - trT2);
+}
+
+static double det3(double t[3][3])
+{
+    double det = 0.0;
+    det += t[0][0] * t[1][1] * t[2][2];
+    det += t[0][1] * t[1][2] * t[2][0];
+    det += t[0][2] * t[1][0] * t[2][1];
+    det -= t[0][2] * t[1][1] * t[2][0];
+    det -= t[0][1] * t[1][0] * t[2][2];
+    det -= t[0][0] * t[1][2] * t[2][1];
+    return det;
+}
+
+/* g: G row-major (g[3 i + j] = d u_i / d x_j); out: QG QS RG RS V2 */
+void orc_invariants_point(const double *g, double *out)
+{
+    double G[3][3], S[3][3], O[3][3], S2[3][3], O2[3][3], S2O2[3][3];
+    for (int i = 0; i <= 2; i++)
+        for (int j = 0; j <= 2; j++) G[i][j] = g[3 * i + j];
+    for (int i = 0; i <= 2; i++)
+        for (int j = 0; j <= 2; j++) S[i][j] = 0.5 * G[i][j] + 0.5 * G[j][i];
+    for (int i = 0; i <= 2; i++)
+        for (int j = 0; j <= 2; j++) O[i][j] = 1.0 * G[i][j] + -1.0 * S[i][j];
+    const double QG = second_inv(G), QS = second_inv(S);
+    mul3(S, S, S2);
+    mul3(O, O, O2);
+    mul3(S2, O2, S2O2);
+    const double QO = QG - QS;
+    out[0] = QG;
+    out[1] = QS;
+    out[2] = det3(G);
+    out[3] = det3(S);
+    out[4] = 4.0 * (tr3(S2O2) - 2.0 * QS * QO);
+}
+
+/* Keep the state at the start of every step (the field whose gradients the
+ * reference's gradu/gradv/gradw hold after a step, src/HIT_LES.cpp:74-125). */
+void orc_keep_start(orc_t *o)
+{
+    for (int c = 0; c < 3; c++)
+        if (!o->uk_start[c]) {
+            o->uk_start[c] = calloc_d(2 * o->nc);
+            memcpy(o->uk_start[c], o->uk[c], sizeof(double) * 2 * o->nc);
+        }
+}
+
+/* Shell spectra of the five invariants (src/HIT_LES.cpp:1297-1319 with the
+ * VelGradInvariants integrands |I_k|^2, :1006-1025): the gradients of the
+ * state at the start of the last step (the current state before any step
+ * or without orc_keep_start), the invariants per point, r2c, normalisation
+ * on D, and the weighted shell sums of Integrate_Field (:1137-1214) over
+ * K = 0..last_rad_max, accumulated in long double.
+ * out: 5 x (last_rad_max + 1), in the order QG QS RG RS V2. */
+void orc_invariant_spectra(orc_t *o, double *out)
+{
+    const int Mx = o->p.Mx, My = o->p.My, Mz = o->p.Mz, nz = o->Mz2 + 1;
+    const int nb = o->last_rad_max + 1;
+    double *const *src = (o->iter > 0 && o->uk_start[0]) ? o->uk_start : o->uk;
+    double *g[9], *inv[5];
+    for (int f = 0; f < 9; f++) g[f] = calloc_d(o->nr);
+    for (int f = 0; f < 5; f++) inv[f] = calloc_d(o->nr);
+    double *ck = calloc_d(2 * o->nc);
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) {
+            gradient_spectrum(o, src[c], d, o->ctmp);
+            c2r3d(&o->fft, o->ctmp, g[3 * c + d], ck);
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < Mx; i++)
+        for (int j = 0; j < My; j++)
+            for (int k = 0; k < Mz; k++) {
+                size_t ind = ((size_t)i * My + j) * 2 * nz + k;
+                double gp[9], v[5];
+                for (int f = 0; f < 9; f++) gp[f] = g[f][ind];
+                orc_invariants_point(gp, v);
+                for (int f = 0; f < 5; f++) inv[f][ind] = v[f];
+            }
+    for (int f = 0; f < 5; f++) {
+        r2c3d(&o->fft, inv[f], ck);
+        normalize_D(o, ck);
+        long double *lb = (long double *)calloc((size_t)nb, sizeof(long double));
+        for (size_t ind = 0; ind < o->nc; ind++) {
+            if (!o->dealiased[ind]) continue;
+            const int K = shell_of((int)o->rad2[ind]);
+            const double aux = ck[2 * ind] * ck[2 * ind] + ck[2 * ind + 1] * ck[2 * ind + 1];
+            lb[K] += o->conjugate[ind] ? (2 * aux) : aux;
+        }
+        for (int K = 0; K < nb; K++) out[(size_t)f * nb + K] = (double)lb[K];
+        free(lb);
+    }
+    for (int f = 0; f < 9; f++) free(g[f]);
+    for (int f = 0; f < 5; f++) free(inv[f]);
+    free(ck);
+}
 
 /* --------------------------- FFT test hooks ----------------------------- */
 int orc_fft_c2r(int Mx, int My, int Mz, const double *cin, double *rout)
