@@ -60,7 +60,9 @@ typedef struct spns_config {
     int32_t use_graph;       /* capture the step as a hipGraph */
     int32_t zpass_variant;   /* z-pass kernel: 0 default (staged inputs; DNS forward folded into the next
                                 pencil's inverse), 1 block per pencil, 2 wave per pencil, 3 staged, not folded */
-    int32_t reserved[5];
+    int32_t gradient_invariants; /* keep the state at the start of each step (one more state copy per step)
+                                    so spns_invariant_spectra can evaluate it */
+    int32_t reserved[4];
 } spns_config;
 
 typedef struct spns_scalars {
@@ -138,6 +140,14 @@ int spns_velocity_physical(spns_ctx *ctx, double *uvw);
 /* HIT::Recalculate_Velocity_Antitransform (src/HIT_LES.cpp:60-69) without
  * copying the field out: refreshes the CFL maxima from the current state. */
 int spns_recalculate_velocity_antitransform(spns_ctx *ctx);
+/* HIT::Recalculate_Invariants_Distribution (src/HIT_LES.cpp:1297-1319) without
+ * the files: shell spectra, K = 0..spns_nbins-1, of the velocity-gradient
+ * invariants QG, QS, RG, RS, V2 (src/tensor_algebra.cpp:250-303) into
+ * spectra[f * nbins + K], f in that order; nbins >= spns_nbins(ctx).  Like
+ * the reference, of the gradients of the field at the start of the last step
+ * (the current field before any step).  Needs config.gradient_invariants;
+ * SPNS_EUNSUPPORTED otherwise. */
+int spns_invariant_spectra(spns_ctx *ctx, int32_t nbins, double *spectra);
 /* Wait for the context's stream. */
 int spns_synchronize(spns_ctx *ctx);
 /* Non-blocking: *done = 1 when every step enqueued by the last spns_step call

@@ -37,7 +37,8 @@ class Config(ctypes.Structure):
         ("force_kmin", ctypes.c_int32), ("force_kmax", ctypes.c_int32),
         ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32),
         ("zpass_variant", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("gradient_invariants", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -54,7 +55,7 @@ EXPORTS = [
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
-    "spns_velocity_physical", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_last_step_ms",
+    "spns_velocity_physical", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_last_step_ms", "spns_invariant_spectra",
     "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
@@ -98,6 +99,7 @@ def load_library(path=LIB_PATH):
         "spns_recalculate_velocity_antitransform": (I32, [P]),
         "spns_synchronize": (I32, [P]),
         "spns_query": (I32, [P, ctypes.POINTER(ctypes.c_int32)]),
+        "spns_invariant_spectra": (I32, [P, I32, D]),
         "spns_last_step_ms": (I32, [P, D]),
         "spns_profile_passes": (I32, [P, I32, D]),
         "spns_fft_create": (I32, [I32, I32, I32, I32, I32, ctypes.POINTER(P)]),
@@ -130,7 +132,7 @@ def _dp(a):
 def make_config(Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag=0.0, C_At=1.0, Lx=1, Ly=1, Lz=1,
                 omega=(0.0, 0.0, 0.0), isNotCubic=False, isSelfAdaptive=False, isCCC=False, precision=PREC_FP64,
                 delta=0.0, fixed_dt=0.0, force_eps=None, force_kmin=1, force_kmax=2, device=0, use_graph=True,
-                zpass_variant=0):
+                zpass_variant=0, gradient_invariants=False):
     """spns_config for the arguments of Solver."""
     Ny = Nx if Ny is None else Ny
     Nz = Nx if Nz is None else Nz
@@ -149,6 +151,7 @@ def make_config(Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag
     c.device = device
     c.use_graph = int(use_graph)
     c.zpass_variant = zpass_variant
+    c.gradient_invariants = int(gradient_invariants)
     return c
 
 
@@ -165,11 +168,11 @@ class Solver:
     def __init__(self, Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag=0.0, C_At=1.0,
                  Lx=1, Ly=1, Lz=1, omega=(0.0, 0.0, 0.0), isNotCubic=False, isSelfAdaptive=False,
                  isCCC=False, precision=PREC_FP64, delta=0.0, fixed_dt=0.0, force_eps=None,
-                 force_kmin=1, force_kmax=2, device=0, use_graph=True, zpass_variant=0):
+                 force_kmin=1, force_kmax=2, device=0, use_graph=True, zpass_variant=0, gradient_invariants=False):
         self.L = load_library()
         c = make_config(Nx, Ny, Nz, Mx, My, Mz, nu, C_Smag, C_At, Lx, Ly, Lz, omega, isNotCubic, isSelfAdaptive,
                         isCCC, precision, delta, fixed_dt, force_eps, force_kmin, force_kmax, device, use_graph,
-                        zpass_variant)
+                        zpass_variant, gradient_invariants)
         Ny = Nx if Ny is None else Ny
         Nz = Nx if Nz is None else Nz
         self.cfg = c
@@ -284,6 +287,13 @@ class Solver:
         self._chk(self.L.spns_complex_conjugate_correction(self.h))
 
     # --- diagnostics ---
+    def invariant_spectra(self):
+        """Shell spectra of QG, QS, RG, RS, V2 of the velocity gradients at the
+        start of the last step: array [5][nbins] (needs gradient_invariants)."""
+        out = np.zeros((5, self.nbins))
+        self._chk(self.L.spns_invariant_spectra(self.h, self.nbins, _dp(out)))
+        return out
+
     def spectrum(self):
         ek = np.zeros(self.nbins)
         tot = ctypes.c_double()

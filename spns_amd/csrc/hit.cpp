@@ -86,6 +86,7 @@ HIT::HIT(const ptrdiff_t& Nx_, const ptrdiff_t& Ny_, const ptrdiff_t& Nz_, const
     c.force_kmax = opt.force_kmax;
     c.device = opt.device;
     c.use_graph = opt.use_graph;
+    c.gradient_invariants = opt.gradient_invariants;
     int rc = spns_create(&c, &ctx);
     if (rc) throw Error(rc, std::string("HIT: ") + spns_last_error(nullptr));
     ActualMx = actual(Mx, Lx); ActualMy = actual(My, Ly); ActualMz = actual(Mz, Lz);
@@ -245,6 +246,23 @@ double HIT::Recalculate_Energy_Cascade(const char* filename)
         for (size_t K = 0; K < Ek.size(); K++) out << K << "\t" << Ek[K] << "\n";
     }
     return Ek_Tot;
+}
+
+// Shell spectra of the five velocity-gradient invariants, one "K\tvalue" file
+// each (src/HIT_LES.cpp:1137-1190 output format), of the gradients at the
+// start of the last step; needs Options::gradient_invariants.
+void HIT::Recalculate_Invariants_Distribution(const char* filename[5])
+{
+    const size_t nb = Ek.size();
+    VelGradInv.assign(5 * nb, 0.0);
+    check(spns_invariant_spectra(ctx, (int32_t)nb, VelGradInv.data()), "Recalculate_Invariants_Distribution");
+    if (!filename) return;
+    for (int f = 0; f < 5; f++) {
+        if (!filename[f]) continue;
+        std::ofstream out(filename[f]);
+        if (!out.is_open()) std::printf("WARNING: HIT::Integrate_Field(): Error opening \"%s\".\n", filename[f]);
+        for (size_t K = 0; K < nb; K++) out << K << "\t" << VelGradInv[f * nb + K] << "\n";
+    }
 }
 
 void HIT::Recalculate_Velocity_Antitransform()

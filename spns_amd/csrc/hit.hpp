@@ -57,6 +57,7 @@ struct Options {
     int k41_ranks = 1;        // emulate the per-rank K41 draws of an n-rank run
     int device = 0;
     bool use_graph = true;
+    bool gradient_invariants = false;  // enables Recalculate_Invariants_Distribution
 };
 
 // HIT (src/HIT_LES.h:21-246).
@@ -85,6 +86,7 @@ public:
     double Recalculate_Energy();                            // :1254-1259
     double Recalculate_Energy_Cascade(const char* filename = nullptr);  // :1261-1266
     void Recalculate_Velocity_Antitransform();              // :60-69
+    void Recalculate_Invariants_Distribution(const char* filename[5] = nullptr);  // :1297-1319
     void FourierVelocity_to_BinaryFile(char const* const filename) const;  // src/HIT_LES_output.cpp:8-10
     void RealVelocity_to_BinaryFile(char const* const filename) const;     // src/HIT_LES_output.cpp:12-14
 
@@ -104,6 +106,9 @@ public:
     const double& getEnstrophy_init_file() const { return Enstrophy_init_file; }
     const double& getEk(ptrdiff_t K) const { return Ek[K]; }
     double getSkewness() const;
+    // shell spectrum K of invariant f (QG, QS, RG, RS, V2) from the last
+    // Recalculate_Invariants_Distribution
+    double getVelGradInv(int f, ptrdiff_t K) const { return VelGradInv[(size_t)f * Ek.size() + K]; }
     spns_ctx* context() const { return ctx; }
 
 private:
@@ -118,7 +123,7 @@ private:
     std::string ascii_name, binary_name;
     double nu, ReLambda = 0.0, Ok_Tot = 0.0, Ek_Tot = 0.0, Ek_init = 0.0, Ek_init_file = 0.0, Enstrophy_init_file = 0.0;
     mutable double At = 0.0, time = 0.0;
-    std::vector<double> Ek;
+    std::vector<double> Ek, VelGradInv;
 };
 
 }  // namespace spns_hit

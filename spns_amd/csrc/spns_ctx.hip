@@ -163,6 +163,8 @@ struct spns_ctx {
     void* X2 = nullptr;
     int64_t x1_slot = 0, x2_slot = 0;
     int n_x1 = 0, nfwd = 3;
+    bool keep_prev = false;           // ukprev holds the start-of-step state
+    double* invspec = nullptr;        // [5][nbins] invariant spectra, then the block partials
     void *twx = nullptr, *twy = nullptr, *twz = nullptr;
     DevScalars* scal = nullptr;
     double* hist_partial = nullptr;
@@ -225,6 +227,7 @@ struct spns_ctx {
         rot = (std::fabs(cfg.omega_x) > 0 || std::fabs(cfg.omega_y) > 0 || std::fabs(cfg.omega_z) > 0);
         sa = cfg.isSelfAdaptive != 0;
         nfwd = les ? 9 : 3;
+        keep_prev = sa || cfg.gradient_invariants;
 
         zkernel = cfg.zpass_variant == 1 ? ZK_BLOCK
                   : cfg.zpass_variant == 2 ? ZK_WAVE
@@ -242,7 +245,7 @@ struct spns_ctx {
         for (int c = 0; c < 3; c++) {
             alloc((void**)&uk[c], sb);
             alloc((void**)&R[c], sb);
-            if (sa) alloc((void**)&ukprev[c], sb);
+            if (keep_prev) alloc((void**)&ukprev[c], sb);
         }
         x1_slot = (int64_t)My * nk1 * nk3p;
         x2_slot = (int64_t)My * Mx * nk3p;
@@ -277,7 +280,7 @@ struct spns_ctx {
         for (int c = 0; c < 3; c++) {
             HIPCHK(hipMemsetAsync(uk[c], 0, sb, st));
             HIPCHK(hipMemsetAsync(R[c], 0, sb, st));
-            if (sa) HIPCHK(hipMemsetAsync(ukprev[c], 0, sb, st));
+            if (keep_prev) HIPCHK(hipMemsetAsync(ukprev[c], 0, sb, st));
         }
         DevScalars s{};
         s.nu = cfg.nu;
@@ -301,7 +304,7 @@ struct spns_ctx {
         for (int c = 0; c < 3; c++) { hipFree(uk[c]); hipFree(R[c]); hipFree(ukprev[c]); }
         hipFree(X1); hipFree(X2); hipFree(twx); hipFree(twy); hipFree(twz);
         hipFree(scal); hipFree(hist_partial); hipFree(om_partial); hipFree(skew_partial); hipFree(mv);
-        hipFree(ek); hipFree(forced); hipFree(ft);
+        hipFree(ek); hipFree(forced); hipFree(ft); hipFree(invspec);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (st) hipStreamDestroy(st);
@@ -309,13 +312,14 @@ struct spns_ctx {
 
     // ---------------------------------------------------------------- passes
     // y passes: rows = y (or i2), batch = (i1 outer, i3 contiguous)
-    void inv_y(bool velocity_only)
+    void inv_y(bool velocity_only, cx<double>* const* src = nullptr)
     {
+        if (!src) src = uk;
         StridedArgs a{};
         int n = 0;
         for (int c = 0; c < 3; c++) {
-            a.job[n++] = StridedJob{uk[c], X1f(c), 0, 0};
-            if (!velocity_only) a.job[n++] = StridedJob{uk[c], X1f(3 + c), 1, 0};
+            a.job[n++] = StridedJob{src[c], X1f(c), 0, 0};
+            if (!velocity_only) a.job[n++] = StridedJob{src[c], X1f(3 + c), 1, 0};
         }
         a.njobs = n;
         a.B = nk3; a.nin = nk2; a.nout = My;
@@ -369,37 +373,79 @@ struct spns_ctx {
         z.phys = phys;
         z.tw = twz;
         if (z.npencil * nk3p > x2_slot || nk3 > nk3p) throw Error(SPNS_EINVAL, "z pass exceeds its buffer");
+        if (mode == 2) {
+            z.kernel = ZK_STAGED_NOFUSE;
+            const hipError_t e = launch_zpass(Mz, prec, mode, z, grid_z, st);
+            if (e == hipErrorNotSupported)
+                throw Error(SPNS_EUNSUPPORTED, "gradient invariants: no staged z pass at this length and precision");
+            HIPCHK(e);
+            return;
+        }
         HIPCHK(launch_zpass(Mz, prec, mode, z, grid_z, st));
     }
 
-    void fwd_x()
+    // Shell spectra of the velocity-gradient invariants QG QS RG RS V2 of the
+    // field at the start of the last step (src/HIT_LES.cpp:1297-1319: the
+    // reference evaluates the gradient arrays its last convective term left,
+    // i.e. of that field; before any step, of the current field).  Runs the
+    // inverse passes on that field, the z pass in its invariants mode (five
+    // fields, forward along z), the forward x and y passes on five fields and
+    // the shell sums.  out: [5][nbins].
+    void invariant_spectra(double* out)
     {
+        if (!cfg.gradient_invariants)
+            throw Error(SPNS_EUNSUPPORTED, "invariant spectra need spns_config.gradient_invariants at creation");
+        const DevScalars sc = read_scalars();
+        cx<double>* const* src = (sc.iter > 0) ? ukprev : uk;
+        inv_y(false, src);
+        inv_x(false);
+        zpass(2, nullptr);
+        fwd_x(5);
+        fwd_y(5);
+        InvSpecArgs a{};
+        for (int f = 0; f < 5; f++) a.f[f] = Cf(f);
+        a.nk1 = nk1; a.nk2 = nk2; a.nk3 = nk3; a.nk3p = nk3p; a.nbins = nbins;
+        a.notcubic = cfg.isNotCubic; a.Lx = cfg.Lx; a.Ly = cfg.Ly; a.Lz = cfg.Lz; a.Mz = Mz;
+        a.inv_nf2 = 1.0 / (nf * nf);
+        const int grid = std::min(1024, nk1 * nk2);
+        // [grid][5][nbins] block partials, then the [5][nbins] sums
+        if (!invspec) alloc((void**)&invspec, sizeof(double) * ((size_t)grid + 1) * 5 * nbins);
+        a.partial = invspec + (size_t)5 * nbins;
+        HIPCHK(launch_inv_spectra(prec, a, grid, invspec, st));
+        HIPCHK(hipMemcpyAsync(out, invspec, sizeof(double) * 5 * nbins, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void fwd_x(int nfields = 0)
+    {
+        const int nf = nfields > 0 ? nfields : nfwd;
         StridedArgs a{};
-        for (int f = 0; f < nfwd; f++) a.job[f] = StridedJob{X2f(f), X1f(f), 0, 0};
-        a.njobs = nfwd;
+        for (int f = 0; f < nf; f++) a.job[f] = StridedJob{X2f(f), X1f(f), 0, 0};
+        a.njobs = nf;
         a.B = nk3; a.nin = Mx; a.nout = nk1;
         a.in_row_stride = nk3p; a.in_a_stride = (int64_t)Mx * nk3p;
         a.out_row_stride = nk3p; a.out_a_stride = (int64_t)nk1 * nk3p;
         a.tw = twx;
         int64_t icap[9], ocap[9];
-        for (int j = 0; j < nfwd; j++) { icap[j] = x2_slot; ocap[j] = x1_slot; }
+        for (int j = 0; j < nf; j++) { icap[j] = x2_slot; ocap[j] = x1_slot; }
         check_strided(a, My, icap, ocap, strided_tile(Mx, prec));
         HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, My, st));
     }
 
-    void fwd_y()
+    void fwd_y(int nfields = 0)
     {
+        const int nf = nfields > 0 ? nfields : nfwd;
         StridedArgs a{};
-        for (int f = 0; f < nfwd; f++) a.job[f] = StridedJob{X1f(f), Cf(f), 0, 0};
-        a.njobs = nfwd;
+        for (int f = 0; f < nf; f++) a.job[f] = StridedJob{X1f(f), Cf(f), 0, 0};
+        a.njobs = nf;
         a.B = nk3; a.nin = My; a.nout = nk2;
         a.in_row_stride = (int64_t)nk1 * nk3p; a.out_row_stride = (int64_t)nk1 * nk3p;
         a.in_a_stride = nk3p; a.out_a_stride = nk3p;
         a.tw = twy;
         int64_t icap[9], ocap[9];
-        for (int j = 0; j < nfwd; j++) { icap[j] = x1_slot; ocap[j] = nmodes_p; }
+        for (int j = 0; j < nf; j++) { icap[j] = x1_slot; ocap[j] = nmodes_p; }
         check_strided(a, nk1, icap, ocap, strided_tile(My, prec));
-        if ((int64_t)nfwd * nmodes_p > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
+        if ((int64_t)nf * nmodes_p > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
         HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, nk1, st));
     }
 
@@ -413,6 +459,7 @@ struct spns_ctx {
         e.notcubic = cfg.isNotCubic; e.Lx = cfg.Lx; e.Ly = cfg.Ly; e.Lz = cfg.Lz; e.Mz = Mz;
         e.mode = mode; e.les = les; e.rot = rot; e.force_on = cfg.force_on;
         e.kfmin = cfg.force_kmin; e.kfmax = cfg.force_kmax; e.sa = sa;
+        e.keepprev = keep_prev && !sa;
         e.nf = nf;
         e.ox = cfg.omega_x; e.oy = cfg.omega_y; e.oz = cfg.omega_z;
         e.scal = scal;
@@ -536,7 +583,7 @@ struct spns_ctx {
     {
         for (int c = 0; c < 3; c++) {
             HIPCHK(hipMemsetAsync(R[c], 0, sizeof(cx<double>) * nmodes_p, st));
-            if (sa) HIPCHK(hipMemsetAsync(ukprev[c], 0, sizeof(cx<double>) * nmodes_p, st));
+            if (keep_prev) HIPCHK(hipMemsetAsync(ukprev[c], 0, sizeof(cx<double>) * nmodes_p, st));
         }
         DevScalars s = read_scalars();
         s.time = 0.0;
@@ -660,7 +707,7 @@ int64_t spns_device_bytes(const spns_config* c)
     const int64_t nk3p = (nk3 + 15) / 16 * 16;
     const int64_t tsz = c->precision == SPNS_PREC_FP32_SPLIT ? 8 : 16;
     const int64_t nfwd = c->C_Smag > 0 ? 9 : 3;
-    const int64_t state = 16 * nk2 * nk1 * nk3p * (c->isSelfAdaptive ? 9 : 6);
+    const int64_t state = 16 * nk2 * nk1 * nk3p * ((c->isSelfAdaptive || c->gradient_invariants) ? 9 : 6);
     const int64_t x1 = std::max<int64_t>(6, nfwd) * My * nk1 * nk3p * tsz;
     const int64_t x2 = 9 * My * Mx * nk3p * tsz;
     return state + x1 + x2 + (64 << 20);  // + partial sums, twiddles, scalars
@@ -996,6 +1043,17 @@ int spns_query(spns_ctx* ctx, int32_t* done)
         HIPCHK(e);
         *done = 1;
     }
+    API_END(ctx)
+}
+
+int spns_invariant_spectra(spns_ctx* ctx, int32_t nbins, double* spectra)
+{
+    API_BEGIN(ctx)
+    if (!spectra || nbins < ctx->nbins) throw Error(SPNS_EINVAL, "spectra must hold 5 x spns_nbins values");
+    std::vector<double> h(5 * (size_t)ctx->nbins);
+    ctx->invariant_spectra(h.data());
+    for (int f = 0; f < 5; f++)
+        for (int K = 0; K < ctx->nbins; K++) spectra[(size_t)f * nbins + K] = h[(size_t)f * ctx->nbins + K];
     API_END(ctx)
 }
 

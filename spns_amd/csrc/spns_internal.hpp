@@ -77,11 +77,20 @@ struct EpiArgs {
     int64_t nmodes;  // padded: nk2 * nk1 * nk3p
     int notcubic, Lx, Ly, Lz, Mz;
     int mode, les, rot, force_on, kfmin, kfmax, sa;
+    int keepprev;  // store the start-of-step state in ukprev (without the kappa-scheme)
     double nf;  // Mx*My*Mz
     double ox, oy, oz;
     const DevScalars* scal;
     double* hist_partial;  // [grid][nbins]
     double* om_partial;    // [grid]
+};
+
+struct InvSpecArgs {
+    const void* f[5];  // compact transformed invariants QG QS RG RS V2 (unnormalised)
+    int nk1, nk2, nk3, nk3p, nbins;
+    int notcubic, Lx, Ly, Lz, Mz;
+    double inv_nf2;    // 1 / (Mx My Mz)^2
+    double* partial;   // [grid][5][nbins]
 };
 
 enum { FIN_REDUCE = 1, FIN_SKEW = 2, FIN_MAXVEL = 4, FIN_DT = 8, FIN_ADVANCE = 16 };
@@ -141,6 +150,7 @@ int zpass_kernel(int L, int prec, const ZArgs& a);              // kernel launch
 hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid, hipStream_t st);
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st);
 hipError_t launch_finalize(const FinArgs& a, hipStream_t st);
+hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* out, hipStream_t st);
 hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st);
 hipError_t launch_forcing(const double* ek, const double* forced, double* ft, int last_eff, const ShellArgs& a, hipStream_t st);
 hipError_t launch_ccc(const ShellArgs& a, hipStream_t st);

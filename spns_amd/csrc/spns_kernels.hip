@@ -456,6 +456,70 @@ __global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
 
 
 // ---------------------------------------------------------------------------
+// Velocity-gradient invariants of g[3 i + j] = d u_i / d x_j, in the order
+// QG QS RG RS V2 (src/tensor_algebra.cpp:250-303, calc_QO :356):
+//   S = 0.5 G + 0.5 G^T, Omega = G - S, Q(T) = 0.5((tr T)^2 - tr(T T)),
+//   R(T) = det T (calc_determinant's term order), V2 = 4(tr(S^2 Omega^2) - 2 QS QO),
+//   QO = QG - QS.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void mul3(const T a[9], const T b[9], T ab[9])
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            T s = (T)0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) s += a[3 * i + k] * b[3 * k + j];
+            ab[3 * i + j] = s;
+        }
+}
+
+template <typename T>
+__device__ __forceinline__ T second_inv(const T a[9])
+{
+    T a2[9];
+    mul3(a, a, a2);
+    const T tr = a[0] + a[4] + a[8], tr2 = a2[0] + a2[4] + a2[8];
+    return (T)0.5 * (tr * tr - tr2);
+}
+
+template <typename T>
+__device__ __forceinline__ T det3(const T t[9])
+{
+    T d = (T)0;
+    d += t[0] * t[4] * t[8];
+    d += t[1] * t[5] * t[6];
+    d += t[2] * t[3] * t[7];
+    d -= t[2] * t[4] * t[6];
+    d -= t[1] * t[3] * t[8];
+    d -= t[0] * t[5] * t[7];
+    return d;
+}
+
+template <typename T>
+__device__ __forceinline__ void grad_invariants(const T g[9], T out[5])
+{
+    T S[9], O[9], S2[9], O2[9], S2O2[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[3 * i + j] = (T)0.5 * g[3 * i + j] + (T)0.5 * g[3 * j + i];
+#pragma unroll
+    for (int i = 0; i < 9; i++) O[i] = g[i] - S[i];
+    const T QG = second_inv(g), QS = second_inv(S);
+    mul3(S, S, S2);
+    mul3(O, O, O2);
+    mul3(S2, O2, S2O2);
+    out[0] = QG;
+    out[1] = QS;
+    out[2] = det3(g);
+    out[3] = det3(S);
+    out[4] = (T)4.0 * ((S2O2[0] + S2O2[4] + S2O2[8]) - (T)2.0 * QS * (QG - QS));
+}
+
+// ---------------------------------------------------------------------------
 // z pass with the next pencil's inputs in flight (the default form).
 // Same arithmetic as k_zpass, restructured around HBM latency:
 //   * the nine half-spectrum source fields of a pencil are staged into LDS
@@ -488,13 +552,13 @@ __host__ __device__ constexpr int zstage_ld()
     // LDS row pitch of the pair buffer, in complex elements: one more than the
     // pairs per point, so row-strided accesses (the physical-space loop, the
     // unpack) spread over all banks
-    return MODE == 0 ? 7 : 3;
+    return MODE == 1 ? 3 : 7;
 }
 
 template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>()>
 __host__ __device__ constexpr size_t zstage_lds_bytes()
 {
-    return sizeof(cx<T>) * ((size_t)L * LD + (size_t)(MODE == 0 ? 9 : 3) * zstage_spc<L>() + L) +
+    return sizeof(cx<T>) * ((size_t)L * LD + (size_t)(MODE == 1 ? 3 : 9) * zstage_spc<L>() + L) +
            sizeof(double) * NT + 2 * 9 * sizeof(void*);
 }
 
@@ -517,8 +581,8 @@ template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int M
           int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2)>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void k_zstage(ZArgs a)
 {
-    static_assert(LD >= (MODE == 0 ? 6 : 2), "pair buffer pitch too small");
-    constexpr int NSRC = (MODE == 0) ? 9 : 3;
+    static_assert(LD >= (MODE == 1 ? 2 : 6), "pair buffer pitch too small");
+    constexpr int NSRC = (MODE == 1) ? 3 : 9;
     constexpr int SPC = zstage_spc<L>();
     constexpr int NW = NT / 64;
     constexpr int EPS = 16 / (int)sizeof(cx<T>);  // elements per 16-byte copy slot
@@ -572,7 +636,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
     for (; pen < a.npencil; pen += gridDim.x) {
         const int64_t off = pen * (int64_t)pitch;
         const int64_t nxt = pen + gridDim.x;
-        constexpr int NPI = (MODE == 0) ? 6 : 2;
+        constexpr int NPI = (MODE == 1) ? 2 : 6;
         const int tl = ZOPAQUE ? opaque(t) : t;
         auto load_inv = [&](int b, int p) -> cx<T> {
             const int nkv = NKC > 0 ? NKC : nk;
@@ -584,7 +648,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
             cx<T> A = stg[fa * SP + k];
             if ((DZ_A >> b) & 1) A = cx<T>{-kk * A.y, kk * A.x};
             cx<T> B = cx<T>{(T)0, (T)0};
-            if (MODE == 0 || b == 0) {
+            if (MODE != 1 || b == 0) {
                 B = stg[fb * SP + k];
                 if ((DZ_B >> b) & 1) B = cx<T>{-kk * B.y, kk * B.x};
             }
@@ -604,6 +668,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
         // -------- physical space
         for (int n = t; n < L; n += NT) {
             cx<T>* row = &fbuf[n * LD];
+            if constexpr (MODE == 2) {
+                // velocity-gradient invariants QG QS RG RS V2 (src/HIT_LES.cpp:1297-1319)
+                const T g[9] = {row[1].y, row[2].x, row[2].y, row[3].x, row[3].y, row[4].x, row[4].y, row[5].x, row[5].y};
+                T q[5];
+                grad_invariants(g, q);
+                row[0] = cx<T>{q[0], q[1]};
+                row[1] = cx<T>{q[2], q[3]};
+                row[2] = cx<T>{q[4], (T)0};
+                continue;
+            }
             const T u = row[0].x, v = row[0].y, w = row[1].x;
             mu = fmax(mu, fabs(u));
             mv = fmax(mv, fabs(v));
@@ -648,7 +722,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
                 }
             }
         }
-        if constexpr (MODE == 0) {
+        if constexpr (MODE == 2) {
+            auto load_lds = [&](int b, int p) -> cx<T> { return fbuf[p * LD + b]; };
+            FftS<T, L, 3, NT, LD, -1, true, TWM>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
+            lds_barrier();
+            dma_wait();
+            for (int idx = t; idx < 3 * nk; idx += NT) {
+                const int q = idx / nk, k = idx - q * nk;
+                const cx<T> Z = fbuf[k * LD + q];
+                const cx<T> Zc = fbuf[((L - k) % L) * LD + q];
+                const T h = (T)0.5;
+                dstp[2 * q][off + k] = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
+                if (q < 2) dstp[2 * q + 1][off + k] = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
+            }
+        } else if constexpr (MODE == 0) {
             auto load_lds = [&](int b, int p) -> cx<T> { return fbuf[p * LD + b]; };
             if (a.les) FftS<T, L, 5, NT, LD, -1, true, TWM>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
             else FftS<T, L, 2, NT, LD, -1, true, TWM>::template run<true>(fbuf, twl, load_lds, store_lds, none, tl);
@@ -671,6 +758,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
         }
         lds_barrier();  // every wave's copies are in; fbuf is free again
     }
+    if constexpr (MODE == 2) return;  // a diagnostic pass: no CFL or skewness partials
     const double bu = block_max<NT>((double)mu, red);
     const double bv = block_max<NT>((double)mv, red);
     const double bw = block_max<NT>((double)mw, red);
@@ -1184,6 +1272,7 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
+                    if (a.keepprev) a.ukprev[c][m] = U[c];  // the start-of-step state
                     N[c].x = U[c].x + dt * (1.5 * R1[c].x - 0.5 * R0[c].x);
                     N[c].y = U[c].y + dt * (1.5 * R1[c].y - 0.5 * R0[c].y);
                 }
@@ -1561,7 +1650,7 @@ constexpr bool zfuse_fits()
 template <typename T, int L, int MODE>
 static int zpass_pick(const ZArgs& a)
 {
-    if constexpr (has_wave_plan<L>::value) {
+    if constexpr (MODE != 2 && has_wave_plan<L>::value) {
         if ((!a.les || MODE == 1) && a.kernel == ZK_WAVE) return ZV_WAVE;
     }
     if (a.kernel == ZK_STAGED || a.kernel == ZK_STAGED_NOFUSE) {
@@ -1612,9 +1701,13 @@ static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
     default:
         break;
     }
-    constexpr int NT = zpass_nt<L>();
-    hipLaunchKernelGGL((k_zpass<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
-    return hipGetLastError();
+    if constexpr (MODE == 2) {
+        return hipErrorNotSupported;  // the invariants pass exists only in the staged kernel
+    } else {
+        constexpr int NT = zpass_nt<L>();
+        hipLaunchKernelGGL((k_zpass<T, L, NT, MODE>), dim3(grid), dim3(NT), 0, st, a);
+        return hipGetLastError();
+    }
 }
 
 template <class K>
@@ -1690,10 +1783,13 @@ hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hip
 {
 #define X(LL)                                                                                  \
     if (L == LL) {                                                                             \
-        if (prec == PREC_FP64) return mode == 0 ? launch_zpass_L<double, LL, 0>(a, grid, st)   \
-                                                : launch_zpass_L<double, LL, 1>(a, grid, st);  \
-        return mode == 0 ? launch_zpass_L<float, LL, 0>(a, grid, st)                           \
-                         : launch_zpass_L<float, LL, 1>(a, grid, st);                          \
+        if (prec == PREC_FP64)                                                                 \
+            return mode == 0   ? launch_zpass_L<double, LL, 0>(a, grid, st)                    \
+                   : mode == 1 ? launch_zpass_L<double, LL, 1>(a, grid, st)                    \
+                               : launch_zpass_L<double, LL, 2>(a, grid, st);                   \
+        return mode == 0   ? launch_zpass_L<float, LL, 0>(a, grid, st)                         \
+               : mode == 1 ? launch_zpass_L<float, LL, 1>(a, grid, st)                         \
+                           : launch_zpass_L<float, LL, 2>(a, grid, st);                        \
     }
     SPNS_FOR_EACH_L(X)
 #undef X
@@ -1724,6 +1820,87 @@ hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid
     return hipErrorInvalidValue;
 }
 
+
+// ---------------------------------------------------------------------------
+// Shell spectra of the five velocity-gradient invariants (src/HIT_LES.cpp:
+// 1297-1319 with the integrands |I_k|^2 of :1006-1025 and the shell sums of
+// Integrate_Field :1137-1214): one wave per block walks (i2, i1) rows of the
+// compact box, lanes along i3; per-block LDS histograms [5][nbins] (fixed
+// order within the wave), then a fixed-order sum over the blocks.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void k_inv_spectra(InvSpecArgs a)
+{
+    // One wave per block, rows (k1, k2) grid-strided.  Along a row the shell
+    // K = round(|k|) never decreases with k3, so each 64-mode chunk splits
+    // into runs of equal K: the first lane of a run sums it in order and adds
+    // it to the block's bins.  Runs of one chunk hit distinct bins and the
+    // chunks follow in program order, so the sums are deterministic (no
+    // atomics); k_inv_sum adds the block partials in block order.
+    extern __shared__ double hs[];  // [5][nbins] bins, [5][64] values, [64] shells
+    const int nb = a.nbins, lane = threadIdx.x;
+    double* vals = hs + 5 * nb;
+    int* kb = (int*)(vals + 5 * 64);
+    for (int i = lane; i < 5 * nb; i += 64) hs[i] = 0.0;
+    __syncthreads();
+    const int nrow = a.nk1 * a.nk2;
+    for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
+        const int i2 = row / a.nk1, i1 = row - i2 * a.nk1;
+        const int k1 = wrap_k(i1, a.nk1), k2 = wrap_k(i2, a.nk2);
+        for (int base = 0; base < a.nk3; base += 64) {
+            const int i3 = base + lane;
+            int K = -1;
+            if (i3 < a.nk3) {
+                const int k3 = i3;
+                K = shell_of(k1 * k1 + k2 * k2 + k3 * k3);
+                const bool skip = a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz));  // src/HIT_LES.cpp:867-871
+                const bool conj = (k3 != 0) && !((a.Mz % 2 == 0) && (k3 == a.Mz / 2));       // src/HIT_LES.cpp:880
+                const double w = skip ? 0.0 : (conj ? 2.0 : 1.0) * a.inv_nf2;
+                const int64_t m = (int64_t)row * a.nk3p + i3;
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    const cx<T> c = ((const cx<T>*)a.f[f])[m];
+                    vals[f * 64 + lane] = w * ((double)c.x * (double)c.x + (double)c.y * (double)c.y);
+                }
+            }
+            kb[lane] = K;
+            __syncthreads();
+            if (K >= 0 && (lane == 0 || kb[lane - 1] != K)) {
+                int e = lane + 1;
+                while (e < 64 && kb[e] == K) e++;
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    double sum = 0.0;
+                    for (int j = lane; j < e; j++) sum += vals[f * 64 + j];
+                    hs[f * nb + K] += sum;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = lane; i < 5 * nb; i += 64) a.partial[(int64_t)blockIdx.x * 5 * nb + i] = hs[i];
+}
+
+static __global__ void k_inv_sum(const double* partial, int nblk, int n, double* out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++) s += partial[(int64_t)b * n + i];
+        out[i] = s;
+    }
+}
+
+hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* out, hipStream_t st)
+{
+    const size_t shm = sizeof(double) * (5 * (size_t)a.nbins + 5 * 64) + sizeof(int) * 64;
+    if (prec == PREC_FP64) hipLaunchKernelGGL((k_inv_spectra<double>), dim3(grid), dim3(64), shm, st, a);
+    else hipLaunchKernelGGL((k_inv_spectra<float>), dim3(grid), dim3(64), shm, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int n = 5 * a.nbins;
+    hipLaunchKernelGGL(k_inv_sum, dim3((n + 255) / 256), dim3(256), 0, st, a.partial, grid, n, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st)
 {

@@ -111,6 +111,7 @@ int main(int argc, char** argv)
         const bool isSA = PM.integer("SelfAdaptiveTimestep", 0) == 1;
         const bool isCCC = PM.integer("ComplexConjugateCorrection", 0) == 1;
         const long Energy_Cascade = PM.integer("Energy_Cascade", 0), Energy_Cascade_Freq = PM.integer("Energy_Cascade_Freq", 0);
+        const long VelGradInv = PM.integer("VelGradInv_Spectra", 0), VelGradInv_Freq = PM.integer("VelGradInv_Spectra_Freq", 0);
         const long Vel_Phys = PM.integer("Velocity_Physical", 0), Vel_Phys_Freq = PM.integer("Velocity_Phys_Freq", 0);
         const long Vel_Four = PM.integer("Velocity_Fourier", 0), Vel_Four_Freq = PM.integer("Velocity_Four_Freq", 0);
         const std::string spec_dir = PM.word("Spectra_Folder", ".");
@@ -130,7 +131,11 @@ int main(int argc, char** argv)
         opt.seed = (unsigned long long)PM.integer("Random_Seed", 0);
         opt.k0 = PM.real("Random_k0", 4.0);
         opt.k41_ranks = (int)PM.integer("K41_Ranks", 1);
+        opt.gradient_invariants = VelGradInv == 1;
         const long max_iter = PM.integer("Max_Iterations", 0);
+        static const char* const inv_names[5] = {"QG", "QS", "RG", "RS", "V2"};
+        char inv_fname[5][1024];
+        const char* inv_ptr[5] = {inv_fname[0], inv_fname[1], inv_fname[2], inv_fname[3], inv_fname[4]};
 
         HIT hit(Nx, Ny, Nz, Mx, My, Mz, nu, C_Smag, C_At, Lx, Ly, Lz, ox, oy, oz,
                 isAsc ? PM.word("ASCII_Input_Filename").c_str() : nullptr,
@@ -155,7 +160,7 @@ int main(int argc, char** argv)
                     Nz, Mx, My, Mz, opt.precision ? "FP32-split" : "FP64", isLES ? ", Smagorinsky" : "",
                     isRotating ? ", rotating" : "");
         char fname[1024];
-        long iter = 0, Ek_file_iter = 0, VelFour_file_iter = 0, VelPhys_file_iter = 0;
+        long iter = 0, Ek_file_iter = 0, VelFour_file_iter = 0, VelPhys_file_iter = 0, Inv_file_iter = 0;
         const auto t0 = std::chrono::steady_clock::now();
         while (hit.gettime() < Final_Time) {  // src/MAIN.cpp:413
             hit.New_Fractional_Step_Method();
@@ -179,6 +184,12 @@ int main(int argc, char** argv)
                 std::snprintf(fname, sizeof fname, "%s/Energy_Cascade_%08ld.dat", spec_dir.c_str(), Ek_file_iter++);
                 hit.Recalculate_Energy_Cascade(fname);
             }
+            if (VelGradInv == 1 && VelGradInv_Freq > 0 && (iter + 1) % VelGradInv_Freq == 0) {  // src/MAIN.cpp:460-469
+                for (int f = 0; f < 5; f++)
+                    std::snprintf(inv_fname[f], sizeof inv_fname[f], "%s/%s_%08ld.dat", spec_dir.c_str(), inv_names[f], Inv_file_iter);
+                Inv_file_iter++;
+                hit.Recalculate_Invariants_Distribution(inv_ptr);
+            }
             if (Vel_Four == 1 && Vel_Four_Freq > 0 && (iter + 1) % Vel_Four_Freq == 0) {
                 std::snprintf(fname, sizeof fname, "%s/Velocity_Fourier_%08ld.bin", four_dir.c_str(), VelFour_file_iter++);
                 hit.FourierVelocity_to_BinaryFile(fname);
@@ -193,6 +204,11 @@ int main(int argc, char** argv)
         if (Energy_Cascade == 1) {
             std::snprintf(fname, sizeof fname, "%s/Energy_Cascade.dat", spec_dir.c_str());
             hit.Recalculate_Energy_Cascade(fname);
+        }
+        if (VelGradInv == 1) {  // src/MAIN.cpp:496-503
+            for (int f = 0; f < 5; f++)
+                std::snprintf(inv_fname[f], sizeof inv_fname[f], "%s/%s.dat", spec_dir.c_str(), inv_names[f]);
+            hit.Recalculate_Invariants_Distribution(inv_ptr);
         }
         if (Vel_Four == 1) {
             std::snprintf(fname, sizeof fname, "%s/Velocity_Fourier.bin", four_dir.c_str());

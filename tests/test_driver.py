@@ -73,3 +73,36 @@ def test_driver_reproduces_qa_log(tmp_path, name):
     assert rows == fx["rows"]
     finals = dict(l.split(":", 1) for l in r.stdout.splitlines() if l.startswith("Final "))
     assert {k.strip(): v.strip() for k, v in finals.items()} == fx["final"]
+
+
+@pytest.mark.gpu
+@needs_run
+def test_driver_writes_invariant_spectra(tmp_path):
+    """VelGradInv_Spectra / VelGradInv_Spectra_Freq (src/MAIN.cpp:311-317,
+    460-469, 496-503): numbered QG QS RG RS V2 files every Freq-th iteration
+    and the unnumbered ones at the end, "K<TAB>value" rows; the final files
+    hold what the library returns for the same seeded run."""
+    import numpy as np
+    import spns_amd
+    params = {"Nx": 16, "Ny": 16, "Nz": 16, "nu": 0.01, "C_At": 0.5, "Final_Time": 100.0, "Max_Iterations": 4,
+              "Random_Seed": 3, "Random_k0": 4.0, "VelGradInv_Spectra": 1, "VelGradInv_Spectra_Freq": 2,
+              "Spectra_Folder": str(tmp_path)}
+    path = write_params(str(tmp_path), params)
+    r = subprocess.run([RUN, path], capture_output=True, text=True, cwd=str(tmp_path), timeout=300)
+    assert r.returncode == 0, r.stderr
+    names = ("QG", "QS", "RG", "RS", "V2")
+    for n in names:
+        for i in (0, 1):
+            assert (tmp_path / ("%s_%08d.dat" % (n, i))).exists()
+        assert not (tmp_path / ("%s_%08d.dat" % (n, 2))).exists()
+    s = spns_amd.Solver(Nx=16, nu=0.01, C_At=0.5, gradient_invariants=True)
+    s.init_random(seed=3, k0=4.0)
+    s.finish_init(False)
+    s.step(4)
+    ref = s.invariant_spectra()
+    s.close()
+    for f, n in enumerate(names):
+        rows = [l.split("\t") for l in (tmp_path / ("%s.dat" % n)).read_text().splitlines()]
+        assert [int(k) for k, _ in rows] == list(range(ref.shape[1]))
+        got = np.array([float(v) for _, v in rows])
+        assert np.allclose(got, ref[f], rtol=1e-5, atol=1e-6 * np.abs(ref[f]).max()), n

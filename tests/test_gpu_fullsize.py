@@ -6,7 +6,8 @@ Checked through properties that hold at any size:
   * the shell spectrum sums to the total energy;
   * time = sum of the steps' dt, all scalars finite;
   * FP32-split E(k), epsilon and skewness agree with the FP64 form
-    (E(k) within the 1e-5 target).
+    (E(k) within the 1e-5 target), and so do the five velocity-gradient
+    invariant spectra (1e-5).
 The same seeded field (k0 = 4, u_rms = 1) starts every run.  State layout:
 [component][i2][i1][i3], the reference's Fourier-dump order."""
 import numpy as np
@@ -45,10 +46,11 @@ def hermitian_rel(st, nk1, nk2, nk3):
     return float(np.linalg.norm(u - mirror) / np.linalg.norm(u))
 
 
-def run(N, prec, steps, seed=2024):
+def run(N, prec, steps, seed=2024, invariants=False):
     import spns_amd
     nu = 8e-5 if N >= 1024 else 2e-4
-    s = spns_amd.Solver(Nx=N, nu=nu, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2, precision=prec)
+    s = spns_amd.Solver(Nx=N, nu=nu, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2, precision=prec,
+                        gradient_invariants=invariants)
     s.init_random(seed=seed, k0=4.0, urms=1.0)
     s.finish_init(False)
     dts = []
@@ -79,12 +81,19 @@ def common_checks(s, sc, ek, dts, N, div_tol, herm_tol, planes=None):
 def test_512_fp64_and_fp32_split():
     import spns_amd
     steps = 10
-    s64, sc64, ek64, dt64 = run(512, spns_amd.PREC_FP64, steps)
+    s64, sc64, ek64, dt64 = run(512, spns_amd.PREC_FP64, steps, invariants=True)
     common_checks(s64, sc64, ek64, dt64, 512, 1e-13, 1e-12)
+    inv64 = s64.invariant_spectra()
     s64.close()
-    s32, sc32, ek32, dt32 = run(512, spns_amd.PREC_FP32_SPLIT, steps)
+    s32, sc32, ek32, dt32 = run(512, spns_amd.PREC_FP32_SPLIT, steps, invariants=True)
     common_checks(s32, sc32, ek32, dt32, 512, 1e-13, 1e-6)
+    inv32 = s32.invariant_spectra()
     s32.close()
+    # velocity-gradient invariant spectra of the same start-of-step field in both forms
+    for f, name in enumerate(("QG", "QS", "RG", "RS", "V2")):
+        e = float(np.linalg.norm(inv32[f] - inv64[f]) / np.linalg.norm(inv64[f]))
+        print("512^3 invariant %s FP32-split vs FP64: %.2e" % (name, e))
+        assert e <= 1e-5, name
     ek_err = float(np.linalg.norm(ek32 - ek64) / np.linalg.norm(ek64))
     eps_err = abs(sc32["nu"] * sc32["om"] - sc64["nu"] * sc64["om"]) / (sc64["nu"] * sc64["om"])
     print("512^3 FP32-split vs FP64 after %d steps: E(k) %.2e eps %.2e skew %.2e" % (

@@ -29,8 +29,8 @@ def rel(a, b):
 
 def pair(seed=1, k0=4.0, **kw):
     g = _solver(**kw)
-    kw.pop("use_graph", None)
-    kw.pop("precision", None)
+    for k in ("use_graph", "precision", "gradient_invariants"):
+        kw.pop(k, None)
     o = Oracle(**kw)
     g.init_random(seed=seed, k0=k0)
     o.set_state(g.get_state())
