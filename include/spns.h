@@ -62,7 +62,9 @@ typedef struct spns_config {
                                 pencil's inverse), 1 block per pencil, 2 wave per pencil, 3 staged, not folded */
     int32_t gradient_invariants; /* keep the state at the start of each step (one more state copy per step)
                                     so spns_invariant_spectra can evaluate it */
-    int32_t reserved[4];
+    int32_t keep_step_start; /* keep it as well (implied by the two flags above) so that
+                                spns_velocity_physical_step_start can give its physical field */
+    int32_t reserved[3];
 } spns_config;
 
 typedef struct spns_scalars {
@@ -137,6 +139,13 @@ int spns_get_scalars(spns_ctx *ctx, spns_scalars *out);
 /* HIT::Recalculate_Velocity_Antitransform + RealVelocity_to_BinaryFile's
  * data (src/HIT_LES.cpp:60-69): u,v,w as [3][Mx][My][Mz] doubles. */
 int spns_velocity_physical(spns_ctx *ctx, double *uvw);
+/* RealVelocity_to_BinaryFile's data between steps (src/HIT_LES_output.cpp:12-14):
+ * the reference's u,v,w arrays then hold the physical field of the state at
+ * the start of the last step (src/HIT_LES.cpp:9-25; the current state before
+ * any step).  Same layout as spns_velocity_physical; the CFL maxima are left
+ * alone.  Needs a kept step-start state (keep_step_start, gradient_invariants
+ * or isSelfAdaptive) once a step has run; SPNS_EUNSUPPORTED otherwise. */
+int spns_velocity_physical_step_start(spns_ctx *ctx, double *uvw);
 /* HIT::Recalculate_Velocity_Antitransform (src/HIT_LES.cpp:60-69) without
  * copying the field out: refreshes the CFL maxima from the current state. */
 int spns_recalculate_velocity_antitransform(spns_ctx *ctx);

@@ -38,7 +38,8 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32),
         ("zpass_variant", ctypes.c_int32),
         ("gradient_invariants", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("keep_step_start", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -55,7 +56,7 @@ EXPORTS = [
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
-    "spns_velocity_physical", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_last_step_ms", "spns_invariant_spectra",
+    "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_last_step_ms", "spns_invariant_spectra",
     "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
@@ -96,6 +97,7 @@ def load_library(path=LIB_PATH):
         "spns_energy_cascade": (I32, [P, D, I32, D]),
         "spns_get_scalars": (I32, [P, ctypes.POINTER(Scalars)]),
         "spns_velocity_physical": (I32, [P, D]),
+        "spns_velocity_physical_step_start": (I32, [P, D]),
         "spns_recalculate_velocity_antitransform": (I32, [P]),
         "spns_synchronize": (I32, [P]),
         "spns_query": (I32, [P, ctypes.POINTER(ctypes.c_int32)]),
@@ -132,7 +134,7 @@ def _dp(a):
 def make_config(Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag=0.0, C_At=1.0, Lx=1, Ly=1, Lz=1,
                 omega=(0.0, 0.0, 0.0), isNotCubic=False, isSelfAdaptive=False, isCCC=False, precision=PREC_FP64,
                 delta=0.0, fixed_dt=0.0, force_eps=None, force_kmin=1, force_kmax=2, device=0, use_graph=True,
-                zpass_variant=0, gradient_invariants=False):
+                zpass_variant=0, gradient_invariants=False, keep_step_start=False):
     """spns_config for the arguments of Solver."""
     Ny = Nx if Ny is None else Ny
     Nz = Nx if Nz is None else Nz
@@ -152,6 +154,7 @@ def make_config(Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag
     c.use_graph = int(use_graph)
     c.zpass_variant = zpass_variant
     c.gradient_invariants = int(gradient_invariants)
+    c.keep_step_start = int(keep_step_start)
     return c
 
 
@@ -168,11 +171,12 @@ class Solver:
     def __init__(self, Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag=0.0, C_At=1.0,
                  Lx=1, Ly=1, Lz=1, omega=(0.0, 0.0, 0.0), isNotCubic=False, isSelfAdaptive=False,
                  isCCC=False, precision=PREC_FP64, delta=0.0, fixed_dt=0.0, force_eps=None,
-                 force_kmin=1, force_kmax=2, device=0, use_graph=True, zpass_variant=0, gradient_invariants=False):
+                 force_kmin=1, force_kmax=2, device=0, use_graph=True, zpass_variant=0, gradient_invariants=False,
+                 keep_step_start=False):
         self.L = load_library()
         c = make_config(Nx, Ny, Nz, Mx, My, Mz, nu, C_Smag, C_At, Lx, Ly, Lz, omega, isNotCubic, isSelfAdaptive,
                         isCCC, precision, delta, fixed_dt, force_eps, force_kmin, force_kmax, device, use_graph,
-                        zpass_variant, gradient_invariants)
+                        zpass_variant, gradient_invariants, keep_step_start)
         Ny = Nx if Ny is None else Ny
         Nz = Nx if Nz is None else Nz
         self.cfg = c
@@ -315,6 +319,14 @@ class Solver:
         Mx, My, Mz = self.shape
         out = np.zeros((3, Mx, My, Mz))
         self._chk(self.L.spns_velocity_physical(self.h, _dp(out)))
+        return out
+
+    def velocity_physical_step_start(self):
+        """Physical field of the state at the start of the last step (what the
+        reference's u,v,w arrays hold between steps); CFL maxima untouched."""
+        Mx, My, Mz = self.shape
+        out = np.zeros((3, Mx, My, Mz))
+        self._chk(self.L.spns_velocity_physical_step_start(self.h, _dp(out)))
         return out
 
 

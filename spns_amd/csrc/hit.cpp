@@ -87,6 +87,7 @@ HIT::HIT(const ptrdiff_t& Nx_, const ptrdiff_t& Ny_, const ptrdiff_t& Nz_, const
     c.device = opt.device;
     c.use_graph = opt.use_graph;
     c.gradient_invariants = opt.gradient_invariants;
+    c.keep_step_start = opt.keep_step_start;
     int rc = spns_create(&c, &ctx);
     if (rc) throw Error(rc, std::string("HIT: ") + spns_last_error(nullptr));
     ActualMx = actual(Mx, Lx); ActualMy = actual(My, Ly); ActualMz = actual(Mz, Lz);
@@ -188,7 +189,11 @@ void HIT::read_input_file(std::vector<double>& u, std::vector<double>& v, std::v
             }
 }
 
-void HIT::New_Fractional_Step_Method() { check(spns_step(ctx, 1), "New_Fractional_Step_Method"); }
+void HIT::New_Fractional_Step_Method()
+{
+    check(spns_step(ctx, 1), "New_Fractional_Step_Method");
+    phys_current = false;  // the step's u,v,w are those of its start state
+}
 void HIT::Recalculate_TimeStep() { check(spns_recalculate_timestep(ctx), "Recalculate_TimeStep"); }
 
 void HIT::Forcing_Energy_Cascade(double const* const Forced_Ek, const ptrdiff_t& last_input_rad,
@@ -270,6 +275,7 @@ void HIT::Recalculate_Velocity_Antitransform()
     // the CFL maximum of the next Recalculate_TimeStep then refers to the
     // current state's physical field, as in the reference
     check(spns_recalculate_velocity_antitransform(ctx), "Recalculate_Velocity_Antitransform");
+    phys_current = true;
 }
 
 double HIT::getSkewness() const
@@ -303,11 +309,17 @@ void HIT::FourierVelocity_to_BinaryFile(char const* const filename) const
 
 // Real3DimField_to_BinaryFile (src/HIT_LES_output.cpp:66-115): u, v, w
 // blocks, each [x][y][z]; non-cubic runs keep x<ActualMx, y<ActualMy, z<ActualMz.
+// The field is what the reference's u,v,w arrays hold: of the current state
+// before any step and after Recalculate_Velocity_Antitransform, else of the
+// last step's start state (needs Options::keep_step_start).
 void HIT::RealVelocity_to_BinaryFile(char const* const filename) const
 {
     const size_t P = (size_t)Mx * My * Mz;
     std::vector<double> uvw(3 * P);
-    check(spns_velocity_physical(ctx, uvw.data()), "RealVelocity_to_BinaryFile");
+    if (phys_current)
+        check(spns_velocity_physical(ctx, uvw.data()), "RealVelocity_to_BinaryFile");
+    else
+        check(spns_velocity_physical_step_start(ctx, uvw.data()), "RealVelocity_to_BinaryFile");
     std::ofstream out(filename, std::ios::binary | std::ios::trunc);
     if (!out) throw Error(SPNS_EINVAL, std::string("cannot open ") + filename);
     for (int c = 0; c < 3; c++)

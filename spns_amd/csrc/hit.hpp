@@ -58,6 +58,7 @@ struct Options {
     int device = 0;
     bool use_graph = true;
     bool gradient_invariants = false;  // enables Recalculate_Invariants_Distribution
+    bool keep_step_start = false;      // RealVelocity_to_BinaryFile between steps (the step-start field)
 };
 
 // HIT (src/HIT_LES.h:21-246).
@@ -123,6 +124,7 @@ private:
     std::string ascii_name, binary_name;
     double nu, ReLambda = 0.0, Ok_Tot = 0.0, Ek_Tot = 0.0, Ek_init = 0.0, Ek_init_file = 0.0, Enstrophy_init_file = 0.0;
     mutable double At = 0.0, time = 0.0;
+    bool phys_current = true;  // u,v,w of the current state (else of the last step's start)
     std::vector<double> Ek, VelGradInv;
 };
 

@@ -227,7 +227,7 @@ struct spns_ctx {
         rot = (std::fabs(cfg.omega_x) > 0 || std::fabs(cfg.omega_y) > 0 || std::fabs(cfg.omega_z) > 0);
         sa = cfg.isSelfAdaptive != 0;
         nfwd = les ? 9 : 3;
-        keep_prev = sa || cfg.gradient_invariants;
+        keep_prev = sa || cfg.gradient_invariants || cfg.keep_step_start;
 
         zkernel = cfg.zpass_variant == 1 ? ZK_BLOCK
                   : cfg.zpass_variant == 2 ? ZK_WAVE
@@ -510,6 +510,21 @@ struct spns_ctx {
     }
 
     // u,v,w from the state: CFL maxima (and the physical field if phys)
+    // Physical velocity of the state at the start of the last step (the
+    // current state before any step) into phys.  The maxima the z pass
+    // accumulates stay in mv, which every step and velocity pass clears and
+    // only a finalize with FIN_MAXVEL reads, so the CFL state is unchanged.
+    void velocity_step_start(void* phys)
+    {
+        const DevScalars sc = read_scalars();
+        if (sc.iter > 0 && !keep_prev)
+            throw Error(SPNS_EUNSUPPORTED, "the step-start field needs spns_config.keep_step_start at creation");
+        cx<double>* const* src = (sc.iter > 0) ? ukprev : uk;
+        inv_y(true, src);
+        inv_x(true);
+        zpass(1, phys);
+    }
+
     void velocity_pass(void* phys)
     {
         HIPCHK(launch_clear(mv, mv + 1, st));
@@ -707,7 +722,7 @@ int64_t spns_device_bytes(const spns_config* c)
     const int64_t nk3p = (nk3 + 15) / 16 * 16;
     const int64_t tsz = c->precision == SPNS_PREC_FP32_SPLIT ? 8 : 16;
     const int64_t nfwd = c->C_Smag > 0 ? 9 : 3;
-    const int64_t state = 16 * nk2 * nk1 * nk3p * ((c->isSelfAdaptive || c->gradient_invariants) ? 9 : 6);
+    const int64_t state = 16 * nk2 * nk1 * nk3p * ((c->isSelfAdaptive || c->gradient_invariants || c->keep_step_start) ? 9 : 6);
     const int64_t x1 = std::max<int64_t>(6, nfwd) * My * nk1 * nk3p * tsz;
     const int64_t x2 = 9 * My * Mx * nk3p * tsz;
     return state + x1 + x2 + (64 << 20);  // + partial sums, twiddles, scalars
@@ -987,9 +1002,10 @@ int spns_get_scalars(spns_ctx* ctx, spns_scalars* out)
     API_END(ctx)
 }
 
-int spns_velocity_physical(spns_ctx* ctx, double* uvw)
+// The velocity pass (current state) or the step-start pass into a device
+// buffer, then [c][y][x][z] copied out as [c][x][y][z] doubles.
+static void physical_out(spns_ctx* ctx, double* uvw, bool step_start)
 {
-    API_BEGIN(ctx)
     if (!uvw) throw Error(SPNS_EINVAL, "null output");
     const int Mx = ctx->Mx, My = ctx->My, Mz = ctx->Mz;
     const size_t P = (size_t)Mx * My * Mz;
@@ -997,7 +1013,10 @@ int spns_velocity_physical(spns_ctx* ctx, double* uvw)
     void* d = nullptr;
     ctx->alloc(&d, 3 * P * ts);
     try {
-        ctx->velocity_pass(d);
+        if (step_start)
+            ctx->velocity_step_start(d);
+        else
+            ctx->velocity_pass(d);
         std::vector<char> hbuf(3 * P * ts);
         HIPCHK(hipMemcpyAsync(hbuf.data(), d, 3 * P * ts, hipMemcpyDeviceToHost, ctx->st));
         HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1014,6 +1033,19 @@ int spns_velocity_physical(spns_ctx* ctx, double* uvw)
         throw;
     }
     hipFree(d);
+}
+
+int spns_velocity_physical(spns_ctx* ctx, double* uvw)
+{
+    API_BEGIN(ctx)
+    physical_out(ctx, uvw, false);
+    API_END(ctx)
+}
+
+int spns_velocity_physical_step_start(spns_ctx* ctx, double* uvw)
+{
+    API_BEGIN(ctx)
+    physical_out(ctx, uvw, true);
     API_END(ctx)
 }
 

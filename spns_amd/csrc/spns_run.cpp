@@ -132,6 +132,7 @@ int main(int argc, char** argv)
         opt.k0 = PM.real("Random_k0", 4.0);
         opt.k41_ranks = (int)PM.integer("K41_Ranks", 1);
         opt.gradient_invariants = VelGradInv == 1;
+        opt.keep_step_start = Vel_Phys == 1 && Vel_Phys_Freq > 0;  // in-loop dumps write the step-start field
         const long max_iter = PM.integer("Max_Iterations", 0);
         static const char* const inv_names[5] = {"QG", "QS", "RG", "RS", "V2"};
         char inv_fname[5][1024];
@@ -214,7 +215,8 @@ int main(int argc, char** argv)
             std::snprintf(fname, sizeof fname, "%s/Velocity_Fourier.bin", four_dir.c_str());
             hit.FourierVelocity_to_BinaryFile(fname);
         }
-        if (Vel_Phys == 1) {
+        if (Vel_Phys == 1) {  // src/MAIN.cpp:510-514: antitransform first, then the current state's field
+            hit.Recalculate_Velocity_Antitransform();
             std::snprintf(fname, sizeof fname, "%s/Velocity.bin", phys_dir.c_str());
             hit.RealVelocity_to_BinaryFile(fname);
         }

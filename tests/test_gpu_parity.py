@@ -137,3 +137,34 @@ def test_qa_replay_on_gpu(name):
     assert len(rows) == len(ref)
     bad = [(i, r, e) for i, (r, e) in enumerate(zip(rows, ref)) if r != e]
     assert len(bad) == 0, bad[:3]
+
+
+@pytest.mark.parametrize("kw", [dict(Nx=32, nu=1e-3, C_At=0.5), dict(Nx=32, nu=1e-4, C_Smag=0.17, C_At=0.5),
+                                dict(Nx=32, nu=1e-3, C_At=0.5, isSelfAdaptive=True)])
+def test_velocity_step_start(kw):
+    """Between steps the physical field of the last step's start state is the
+    field of the state before that step, bit for bit, and fetching it leaves
+    the CFL step alone (Recalculate_TimeStep gives the same dt)."""
+    import spns_amd
+    g = _solver(keep_step_start=True, **kw)
+    g.init_random(seed=4, k0=4.0)
+    g.finish_init(False)
+    assert np.array_equal(g.velocity_physical_step_start(), g.velocity_physical())
+    for _ in range(3):
+        before = g.velocity_physical()
+        g.step(1)
+        g.recalc_timestep()
+        dt = g.scalars()["dt"]
+        got = g.velocity_physical_step_start()
+        g.recalc_timestep()
+        assert g.scalars()["dt"] == dt
+        assert np.array_equal(got, before)
+    h = _solver(**kw)
+    h.init_random(seed=4, k0=4.0)
+    h.finish_init(False)
+    h.step(1)
+    if not kw.get("isSelfAdaptive"):
+        with pytest.raises(spns_amd.SpnsError):
+            h.velocity_physical_step_start()
+    g.close()
+    h.close()
