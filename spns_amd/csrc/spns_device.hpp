@@ -15,9 +15,24 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "spns_internal.hpp"
 
 namespace spns {
+
+// f(std::integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_seq(F& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T a, T b) { return cx<T>{a, b}; }
@@ -417,6 +432,8 @@ __host__ __device__ constexpr int fft_twiddle_span()
 // (b = idx / (L/R0), j = idx % (L/R0)) instead of across columns, so with
 // L/R0 a multiple of 64 every wave of stage 0 works on a single column and a
 // column-dependent loader runs without divergence.
+// Stage 0 calls load(b, j, std::integral_constant<int, r>) for input
+// j + r*(L/R0) when the functor accepts that form, else load(b, j + r*(L/R0)).
 template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false, int TWM = (sizeof(T) == 8 ? 1 : 2),
           bool S0COL = false>
 struct FftS {
@@ -443,10 +460,17 @@ struct FftS {
                 const int j = (S == 0 && S0COL) ? idx % LR : idx / NB;
                 __builtin_assume(j >= 0 && j < LR);  // idx < NBF = LR * NB
                 __builtin_assume(b >= 0 && b < NB);
+                if constexpr (S == 0 && std::is_invocable_v<Load&, int, int, std::integral_constant<int, 0>>) {
+                    // load(b, j, r as a compile-time constant): the loader can
+                    // resolve per-input structure (zero padding, mirroring) statically
+                    auto ld = [&](auto rc) { v[q][decltype(rc)::value] = load(b, j, rc); };
+                    static_for<R>(ld);
+                } else {
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    if constexpr (S == 0) v[q][r] = load(b, j + r * LR);
-                    else v[q][r] = lds[(j + r * LR) * LD + b];
+                    for (int r = 0; r < R; r++) {
+                        if constexpr (S == 0) v[q][r] = load(b, j + r * LR);
+                        else v[q][r] = lds[(j + r * LR) * LD + b];
+                    }
                 }
             }
         }

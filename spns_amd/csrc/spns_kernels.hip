@@ -818,7 +818,6 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
     __shared__ __align__(16) cx<T> stg[9 * SPC];
     __shared__ __align__(16) cx<T> twl[TWN];
     __shared__ const cx<T>* srcp[9];
-    __shared__ cx<T>* dstp[3];
     const int t = threadIdx.x;
     const int nk = a.nk3, pitch = a.pitch;
     const int SP = pitch + zstage_pad<T>();
@@ -829,23 +828,37 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
     const bool sa = a.sa;
 
     if (t < 9) srcp[t] = (const cx<T>*)a.f[t];
-    if (t < 3) dstp[t] = (cx<T>*)a.f[t];
     for (int i = t; i < TWN; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
     __syncthreads();
 
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
     const uint32_t stg_base = lds_addr(stg);
+    // Staging copies: slot m (16 bytes per lane) of wave m % NW.  Each lane's
+    // source pointer (field f, k3 = k) is the same for every pencil up to the
+    // pencil offset, so it is formed once here.
+    constexpr int MAXM = (9 * (SPC + zstage_pad<T>()) / EPS + 64 * NW - 1) / (64 * NW);
+    const cx<T>* gsrc[MAXM];
+    bool gok[MAXM];
+#pragma unroll
+    for (int i = 0; i < MAXM; i++) {
+        const int m = wave + i * NW;
+        const int s = m * 64 + lane;
+        const int e = s * EPS;
+        const int f = e / SP, k = e - f * SP;
+        gok[i] = s < nslot && k < pitch;
+        gsrc[i] = srcp[gok[i] ? f : 0] + (gok[i] ? k : 0);
+    }
     auto issue = [&](int64_t pen) {
         const int64_t off = pen * (int64_t)pitch;
-        for (int m = wave; m * 64 < nslot; m += NW) {
-            const int s = m * 64 + lane;
-            const int e = s * EPS;
-            const int f = e / SP, k = e - f * SP;
-            if (s < nslot && k < pitch) dma16(srcp[f] + off + k, stg_base + (uint32_t)m * 1024u);
+#pragma unroll
+        for (int i = 0; i < MAXM; i++) {
+            const int m = wave + i * NW;
+            if (m * 64 < nslot && gok[i]) dma16(gsrc[i] + off, stg_base + (uint32_t)m * 1024u);
         }
     };
     // inverse pair b (0..5): logical fields (2b, 2b+1) of u v w ux uy uz vx vy vz wx wy wz
     constexpr uint32_t SRC_A = 0x814620u, SRC_B = 0x257031u, DZ_A = 0x10u, DZ_B = 0x24u;
+    constexpr bool FAST = sizeof(T) == 4 && NKC > 0 && L / Plan<L>::r(0) == 64 && NB * 64 <= NT;
 
     int64_t pen = blockIdx.x;
     if (pen < a.npencil) issue(pen);
@@ -880,23 +893,77 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
             const T by = (p == 0) ? (T)0 : s * B.y;
             return cx<T>{A.x - by, ay + B.x};
         };
+        // FP32 fast loader.  Stage 0 is column-major with L/R0 = 64 and one
+        // butterfly per thread, so column b is the wave index (a scalar) and j
+        // the lane; the zone of input p = 64 r + j (direct k3 = p, mirrored
+        // k3 = L - p, or zero padding) is fixed for each r at compile time.
+        // Columns pair each velocity component with its own z derivative,
+        // (f, i k f) -> (1 - k) f direct and (1 + k) conj f mirrored (one
+        // read, no derivative product), then (fx, fy) of u, v, w.
+        const cx<T>* pa = stg + wave * SP;        // f (columns 0-2) or fx (3-5)
+        const cx<T>* pb = stg + (wave + 3) * SP;  // fy (columns 3-5)
+        auto load_fast = [&](int, int j, auto rc) -> cx<T> {
+            constexpr int r = decltype(rc)::value;
+            constexpr int p0 = 64 * r;
+            if (wave >= 6) {
+                if (!havep) return cx<T>{(T)0, (T)0};
+                const cx<T> v = fbuf[(p0 + j) * LD + wave];
+                return cx<T>{v.x, -v.y};  // forward = conj(inverse(conj x))
+            }
+            if (!have) return cx<T>{(T)0, (T)0};
+            constexpr bool LO = p0 + 63 < NKC, HI = p0 > L - NKC, ZERO = p0 >= NKC && p0 + 63 <= L - NKC;
+            if constexpr (ZERO) {
+                return cx<T>{(T)0, (T)0};
+            } else {
+                const int p = p0 + j;
+                bool lo = LO, hi = HI;
+                if constexpr (!LO && !HI) {
+                    lo = p < NKC;
+                    hi = p > L - NKC;
+                    if (!(lo || hi)) return cx<T>{(T)0, (T)0};
+                }
+                const int k = lo ? p : L - p;
+                const cx<T> A = pa[k];
+                T ay = lo ? A.y : -A.y;
+                if constexpr (r == 0) {
+                    if (j == 0) ay = (T)0;  // the k3 = 0 imaginary part is dropped, as by a c2r
+                }
+                if (wave < 3) {
+                    const T sc = lo ? (T)1 - (T)k : (T)1 + (T)k;
+                    return cx<T>{sc * A.x, sc * ay};
+                }
+                const cx<T> B = pb[k];
+                T by = lo ? B.y : -B.y;
+                if constexpr (r == 0) {
+                    if (j == 0) by = (T)0;
+                }
+                return cx<T>{A.x - by, ay + B.x};
+            }
+        };
         auto store = [&](int b, int p, cx<T> v) { fbuf[p * LD + b] = (b >= 6) ? cx<T>{v.x, -v.y} : v; };
         auto prefetch = [&]() {
             if (have && nxt < a.npencil) issue(nxt);
         };
-        FftS<T, L, NB, NT, LD, +1, true, TWM, sizeof(T) == 4>::template run<false>(fbuf, twl, load, store, prefetch, tl);
+        if constexpr (FAST)
+            FftS<T, L, NB, NT, LD, +1, true, TWM, true>::template run<false>(fbuf, twl, load_fast, store, prefetch, tl);
+        else
+            FftS<T, L, NB, NT, LD, +1, true, TWM, sizeof(T) == 4>::template run<false>(fbuf, twl, load, store, prefetch, tl);
         lds_barrier();
         dma_wait();  // pencil n+1's sources (this wave's copies) have landed
         if (havep) {
             // -------- unpack pencil n-1: (cx, cy) and (cz, -) half spectra
             const int64_t offp = prev * (int64_t)pitch;
-            for (int idx = t; idx < 2 * nk; idx += NT) {
-                const int q = idx >= nk ? 1 : 0, k = idx - q * nk;
+            const int nkv = NKC > 0 ? NKC : nk;
+            cx<T>* const d0 = (cx<T>*)a.f[0] + offp;
+            cx<T>* const d1 = (cx<T>*)a.f[1] + offp;
+            cx<T>* const d2 = (cx<T>*)a.f[2] + offp;
+            for (int idx = t; idx < 2 * nkv; idx += NT) {
+                const int q = idx >= nkv ? 1 : 0, k = idx - q * nkv;
                 const cx<T> Z = fbuf[k * LD + 6 + q];
-                const cx<T> Zc = fbuf[((L - k) % L) * LD + 6 + q];
+                const cx<T> Zc = fbuf[(k == 0 ? 0 : L - k) * LD + 6 + q];
                 const T h = (T)0.5;
-                dstp[2 * q][offp + k] = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
-                if (q == 0) dstp[1][offp + k] = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
+                (q ? d2 : d0)[k] = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
+                if (q == 0) d1[k] = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
             }
             lds_barrier();
         }
@@ -904,14 +971,20 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
             // -------- physical space of pencil n: products into columns 6-7
             for (int n = t; n < L; n += NT) {
                 cx<T>* row = &fbuf[n * LD];
-                const T u = row[0].x, v = row[0].y, w = row[1].x;
+                T u, v, w, ux, uy, uz, vx, vy, vz, wx, wy, wz;
+                if constexpr (FAST) {
+                    u = row[0].x, uz = row[0].y, v = row[1].x, vz = row[1].y, w = row[2].x, wz = row[2].y;
+                    ux = row[3].x, uy = row[3].y, vx = row[4].x, vy = row[4].y, wx = row[5].x, wy = row[5].y;
+                } else {
+                    u = row[0].x, v = row[0].y, w = row[1].x;
+                    ux = row[1].y, uy = row[2].x, uz = row[2].y;
+                    vx = row[3].x, vy = row[3].y, vz = row[4].x;
+                    wx = row[4].y, wy = row[5].x, wz = row[5].y;
+                }
                 mu = fmax(mu, fabs(u));
                 mv = fmax(mv, fabs(v));
                 mw = fmax(mw, fabs(w));
                 if (sa) vmax_sa = fmax(vmax_sa, fabs((double)u) / iAx + fabs((double)v) / iAy + fabs((double)w) / iAz);  // src/HIT_LES.cpp:377
-                const T ux = row[1].y, uy = row[2].x, uz = row[2].y;
-                const T vx = row[3].x, vy = row[3].y, vz = row[4].x;
-                const T wx = row[4].y, wy = row[5].x, wz = row[5].y;
                 s2 += (double)ux * (double)ux;
                 s3 += (double)ux * (double)ux * (double)ux;
                 // advective form (src/HIT_LES.cpp:118-120)
