@@ -317,10 +317,8 @@ struct spns_ctx {
         if (!src) src = uk;
         StridedArgs a{};
         int n = 0;
-        for (int c = 0; c < 3; c++) {
-            a.job[n++] = StridedJob{src[c], X1f(c), 0, 0};
-            if (!velocity_only) a.job[n++] = StridedJob{src[c], X1f(3 + c), 1, 0};
-        }
+        for (int c = 0; c < 3; c++)  // u_c, and d/dy u_c from the same read of u_c
+            a.job[n++] = StridedJob{src[c], X1f(c), 0, 0, velocity_only ? nullptr : X1f(3 + c)};
         a.njobs = n;
         a.B = nk3; a.nin = nk2; a.nout = My;
         a.in_row_stride = (int64_t)nk1 * nk3p; a.out_row_stride = (int64_t)nk1 * nk3p;
@@ -337,12 +335,9 @@ struct spns_ctx {
     {
         StridedArgs a{};
         int n = 0;
-        for (int c = 0; c < 3; c++) {
-            a.job[n++] = StridedJob{X1f(c), X2f(c), 0, 0};
-            if (!velocity_only) {
-                a.job[n++] = StridedJob{X1f(c), X2f(3 + c), 1, 0};
-                a.job[n++] = StridedJob{X1f(3 + c), X2f(6 + c), 0, 0};
-            }
+        for (int c = 0; c < 3; c++) {  // u_c and d/dx u_c from one read, then d/dy u_c
+            a.job[n++] = StridedJob{X1f(c), X2f(c), 0, 0, velocity_only ? nullptr : X2f(3 + c)};
+            if (!velocity_only) a.job[n++] = StridedJob{X1f(3 + c), X2f(6 + c), 0, 0};
         }
         a.njobs = n;
         a.B = nk3; a.nin = nk1; a.nout = Mx;

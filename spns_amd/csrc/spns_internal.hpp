@@ -27,6 +27,7 @@ struct StridedJob {
     void* out;
     int deriv;  // 1: multiply the input by i*k of the transformed dimension
     int pad;
+    void* out2;  // if set: out gets the plain transform and out2 the i*k one, from one read of in
 };
 
 struct StridedArgs {

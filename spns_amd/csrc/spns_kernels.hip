@@ -188,7 +188,8 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
         const int64_t ai = it / per_a;
         const int r = (int)(it - ai * per_a);
         const int jb = r / ntiles, b0 = (r - jb * ntiles) * TB;
-        const int deriv = a.job[jb].deriv;
+        const bool dual = a.job[jb].out2 != nullptr;
+        int deriv = dual ? 0 : a.job[jb].deriv;
         const int64_t nxt = it + G;
         auto load = [&](int b, int p) -> cx<T> {
             int i, k;
@@ -208,18 +209,32 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
         auto prefetch = [&]() {
             if (nxt < nitems) issue(nxt);
         };
-        FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
+        auto write_tile = [&](void* dst) {
+            cx<TOUT>* out = (cx<TOUT>*)dst + ai * a.out_a_stride + b0;
+            for (int idx = t; idx < a.nout * TB; idx += NT) {
+                const int i = idx / TB, b = idx - i * TB;
+                const int p = (a.nout == L || i <= hout) ? i : L - (a.nout - i);
+                if (b0 + b < a.B) {
+                    const cx<T> v = fbuf[p * TB + b];
+                    out[(int64_t)i * a.out_row_stride + b] = cx<TOUT>{(TOUT)v.x, (TOUT)v.y};
+                }
+            }
+        };
+        if (dual) {
+            // plain transform first, from the staged tile; the tile stays
+            // staged (no prefetch yet) for the i k transform below
+            auto none = [] {};
+            FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<false>(fbuf, twl, load, store_lds, none, t);
+            lds_barrier();
+            write_tile(a.job[jb].out);
+            deriv = 1;
+            FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<true>(fbuf, twl, load, store_lds, prefetch, t);
+        } else {
+            FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
+        }
         lds_barrier();
         dma_wait();  // the next item's tile (this wave's copies) has landed
-        cx<TOUT>* out = (cx<TOUT>*)a.job[jb].out + ai * a.out_a_stride + b0;
-        for (int idx = t; idx < a.nout * TB; idx += NT) {
-            const int i = idx / TB, b = idx - i * TB;
-            const int p = (a.nout == L || i <= hout) ? i : L - (a.nout - i);
-            if (b0 + b < a.B) {
-                const cx<T> v = fbuf[p * TB + b];
-                out[(int64_t)i * a.out_row_stride + b] = cx<TOUT>{(TOUT)v.x, (TOUT)v.y};
-            }
-        }
+        write_tile(dual ? a.job[jb].out2 : a.job[jb].out);
         lds_barrier();  // every wave's copies are in; fbuf is free again
     }
 }
@@ -900,14 +915,15 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
         // Columns pair each velocity component with its own z derivative,
         // (f, i k f) -> (1 - k) f direct and (1 + k) conj f mirrored (one
         // read, no derivative product), then (fx, fy) of u, v, w.
-        const cx<T>* pa = stg + wave * SP;        // f (columns 0-2) or fx (3-5)
-        const cx<T>* pb = stg + (wave + 3) * SP;  // fy (columns 3-5)
+        const int col = wave;  // this thread's stage-0 column
+        const cx<T>* pa = stg + col * SP;        // f (columns 0-2) or fx (3-5)
+        const cx<T>* pb = stg + (col + 3) * SP;  // fy (columns 3-5)
         auto load_fast = [&](int, int j, auto rc) -> cx<T> {
             constexpr int r = decltype(rc)::value;
             constexpr int p0 = 64 * r;
-            if (wave >= 6) {
+            if (col >= 6) {
                 if (!havep) return cx<T>{(T)0, (T)0};
-                const cx<T> v = fbuf[(p0 + j) * LD + wave];
+                const cx<T> v = fbuf[(p0 + j) * LD + col];
                 return cx<T>{v.x, -v.y};  // forward = conj(inverse(conj x))
             }
             if (!have) return cx<T>{(T)0, (T)0};
@@ -928,7 +944,7 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
                 if constexpr (r == 0) {
                     if (j == 0) ay = (T)0;  // the k3 = 0 imaginary part is dropped, as by a c2r
                 }
-                if (wave < 3) {
+                if (col < 3) {
                     const T sc = lo ? (T)1 - (T)k : (T)1 + (T)k;
                     return cx<T>{sc * A.x, sc * ay};
                 }
@@ -1610,6 +1626,19 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
             return hipGetLastError();
         }
     }
+    // the direct-load kernel takes single jobs: a dual job becomes two
+    int n = 0;
+    for (int j = 0; j < a.njobs; j++) {
+        if (a.job[j].out2) {
+            if (n + 2 > 9) return hipErrorInvalidValue;
+            b.job[n++] = StridedJob{a.job[j].in, a.job[j].out, 0, 0, nullptr};
+            b.job[n++] = StridedJob{a.job[j].in, a.job[j].out2, 1, 0, nullptr};
+        } else {
+            if (n + 1 > 9) return hipErrorInvalidValue;
+            b.job[n++] = a.job[j];
+        }
+    }
+    b.njobs = n;
     constexpr int NT = 256;
     const int64_t blocks = (int64_t)((a.B + TB - 1) / TB) * A;
     dim3 grid((unsigned)((blocks + 7) / 8 * 8), 1, 1);
