@@ -873,7 +873,10 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
     };
     // inverse pair b (0..5): logical fields (2b, 2b+1) of u v w ux uy uz vx vy vz wx wy wz
     constexpr uint32_t SRC_A = 0x814620u, SRC_B = 0x257031u, DZ_A = 0x10u, DZ_B = 0x24u;
-    constexpr bool FAST = sizeof(T) == 4 && NKC > 0 && L / Plan<L>::r(0) == 64 && NB * 64 <= NT;
+    // compile-time stage-0 loader: needs the compile-time k3 count, L/R0 = 64
+    // and at most one stage-0 butterfly per thread
+    constexpr bool FAST = NKC > 0 && L / Plan<L>::r(0) == 64 && NB * 64 <= NT;
+    constexpr bool S0COL = sizeof(T) == 4;  // column-major stage 0 in FP32 (its writes conflict in FP64)
 
     int64_t pen = blockIdx.x;
     if (pen < a.npencil) issue(pen);
@@ -908,14 +911,15 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
             const T by = (p == 0) ? (T)0 : s * B.y;
             return cx<T>{A.x - by, ay + B.x};
         };
-        // FP32 fast loader.  Stage 0 is column-major with L/R0 = 64 and one
-        // butterfly per thread, so column b is the wave index (a scalar) and j
-        // the lane; the zone of input p = 64 r + j (direct k3 = p, mirrored
-        // k3 = L - p, or zero padding) is fixed for each r at compile time.
-        // Columns pair each velocity component with its own z derivative,
-        // (f, i k f) -> (1 - k) f direct and (1 + k) conj f mirrored (one
-        // read, no derivative product), then (fx, fy) of u, v, w.
-        const int col = wave;  // this thread's stage-0 column
+        // Fast loader.  With L/R0 = 64 and one stage-0 butterfly per thread,
+        // input p = 64 r + j with j < 64, so the zone of p (direct k3 = p,
+        // mirrored k3 = L - p, or zero padding) is fixed for each r at compile
+        // time.  The thread's column is the wave index (a scalar) when stage 0
+        // is column-major, else t mod 8.  Columns pair each velocity component
+        // with its own z derivative, (f, i k f) -> (1 - k) f direct and
+        // (1 + k) conj f mirrored (one read, no derivative product), then
+        // (fx, fy) of u, v, w.
+        const int col = S0COL ? wave : (tl & (NB - 1));
         const cx<T>* pa = stg + col * SP;        // f (columns 0-2) or fx (3-5)
         const cx<T>* pb = stg + (col + 3) * SP;  // fy (columns 3-5)
         auto load_fast = [&](int, int j, auto rc) -> cx<T> {
@@ -961,9 +965,9 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
             if (have && nxt < a.npencil) issue(nxt);
         };
         if constexpr (FAST)
-            FftS<T, L, NB, NT, LD, +1, true, TWM, true>::template run<false>(fbuf, twl, load_fast, store, prefetch, tl);
+            FftS<T, L, NB, NT, LD, +1, true, TWM, S0COL>::template run<false>(fbuf, twl, load_fast, store, prefetch, tl);
         else
-            FftS<T, L, NB, NT, LD, +1, true, TWM, sizeof(T) == 4>::template run<false>(fbuf, twl, load, store, prefetch, tl);
+            FftS<T, L, NB, NT, LD, +1, true, TWM, S0COL>::template run<false>(fbuf, twl, load, store, prefetch, tl);
         lds_barrier();
         dma_wait();  // pencil n+1's sources (this wave's copies) have landed
         if (havep) {
