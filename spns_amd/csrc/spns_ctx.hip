@@ -175,6 +175,7 @@ struct spns_ctx {
     double* forced = nullptr;
     double* ft = nullptr;
     int grid_epi = 0, grid_z = 0;
+    int grid_yepi = 0;  // > 0: the DNS step fuses its forward y pass with the epilogue (k_yepi)
     int zkernel = ZK_STAGED;  // z-pass kernel (environment SPNS_ZPASS=staged|nofuse|block|wave overrides)
     hipGraphExec_t graph = nullptr;
     bool graph_ready = false;
@@ -281,6 +282,18 @@ struct spns_ctx {
             HIPCHK(hipMemsetAsync(uk[c], 0, sb, st));
             HIPCHK(hipMemsetAsync(R[c], 0, sb, st));
             if (keep_prev) HIPCHK(hipMemsetAsync(ukprev[c], 0, sb, st));
+        }
+        // fused forward y pass + epilogue for DNS steps (environment
+        // SPNS_EPI=split keeps the two kernels apart)
+        {
+            const char* ev = std::getenv("SPNS_EPI");
+            const bool split = ev && std::string(ev) == "split";
+            grid_yepi = 0;
+            if (!les && !split && fwd_y_args(3).padded_ok) {
+                const int tb = strided_tile(My, prec);
+                const int64_t items = tb > 0 ? (int64_t)nk1 * ((nk3 + tb - 1) / tb) : 0;
+                if (items > 0) grid_yepi = yepi_grid(My, prec, nbins, nk2, items, grid_epi);
+            }
         }
         DevScalars s{};
         s.nu = cfg.nu;
@@ -427,9 +440,8 @@ struct spns_ctx {
         HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, My, st));
     }
 
-    void fwd_y(int nfields = 0)
+    StridedArgs fwd_y_args(int nf)
     {
-        const int nf = nfields > 0 ? nfields : nfwd;
         StridedArgs a{};
         for (int f = 0; f < nf; f++) a.job[f] = StridedJob{X1f(f), Cf(f), 0, 0};
         a.njobs = nf;
@@ -441,7 +453,27 @@ struct spns_ctx {
         for (int j = 0; j < nf; j++) { icap[j] = x1_slot; ocap[j] = nmodes_p; }
         check_strided(a, nk1, icap, ocap, strided_tile(My, prec));
         if ((int64_t)nf * nmodes_p > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
+        return a;
+    }
+
+    void fwd_y(int nfields = 0)
+    {
+        const StridedArgs a = fwd_y_args(nfields > 0 ? nfields : nfwd);
         HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, nk1, st));
+    }
+
+    // the DNS step's forward y pass fused with its epilogue (k_yepi) when
+    // grid_yepi > 0, else fwd_y then k_epilogue; returns the block count
+    // of the shell partials the finalize sums
+    int fwd_y_epilogue()
+    {
+        if (grid_yepi > 0) {
+            HIPCHK(launch_yepi(My, prec, fwd_y_args(3), nk1, epi_args(EPI_STEP), grid_yepi, st));
+            return grid_yepi;
+        }
+        fwd_y();
+        HIPCHK(launch_epilogue(prec, epi_args(EPI_STEP), grid_epi, st));
+        return grid_epi;
     }
 
     EpiArgs epi_args(int mode)
@@ -493,9 +525,9 @@ struct spns_ctx {
         inv_x(false);
         zpass(0, nullptr);
         fwd_x();
-        fwd_y();
-        HIPCHK(launch_epilogue(prec, epi_args(EPI_STEP), grid_epi, st));
-        HIPCHK(launch_finalize(fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE), st));
+        FinArgs f = fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE);
+        f.nblk_epi = fwd_y_epilogue();
+        HIPCHK(launch_finalize(f, st));
     }
 
     void reduce_state()
@@ -1105,11 +1137,18 @@ int spns_profile_passes(spns_ctx* ctx, int32_t nsteps, double* ms)
             HIPCHK(hipEventRecord(ev[4], ctx->st));
             ctx->fwd_x();
             HIPCHK(hipEventRecord(ev[5], ctx->st));
-            ctx->fwd_y();
-            HIPCHK(hipEventRecord(ev[6], ctx->st));
-            HIPCHK(launch_epilogue(ctx->prec, ctx->epi_args(EPI_STEP), ctx->grid_epi, ctx->st));
+            FinArgs f = ctx->fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE);
+            if (ctx->grid_yepi > 0) {
+                // fused: the whole kernel is timed as fwd_y, the epilogue slot stays 0
+                f.nblk_epi = ctx->fwd_y_epilogue();
+                HIPCHK(hipEventRecord(ev[6], ctx->st));
+            } else {
+                ctx->fwd_y();
+                HIPCHK(hipEventRecord(ev[6], ctx->st));
+                HIPCHK(launch_epilogue(ctx->prec, ctx->epi_args(EPI_STEP), ctx->grid_epi, ctx->st));
+            }
             HIPCHK(hipEventRecord(ev[7], ctx->st));
-            HIPCHK(launch_finalize(ctx->fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE), ctx->st));
+            HIPCHK(launch_finalize(f, ctx->st));
             HIPCHK(hipEventRecord(ev[8], ctx->st));
             HIPCHK(hipEventSynchronize(ev[8]));
             for (int i = 0; i < NP; i++) {

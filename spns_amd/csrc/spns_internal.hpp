@@ -149,6 +149,8 @@ hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hip
 int zpass_grid(int L, int prec, const ZArgs& a, int64_t npen);  // < 0: no such length / HIP error
 int zpass_kernel(int L, int prec, const ZArgs& a);              // kernel launch_zpass picks
 hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid, hipStream_t st);
+int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid);
+hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st);
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st);
 hipError_t launch_finalize(const FinArgs& a, hipStream_t st);
 hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* out, hipStream_t st);

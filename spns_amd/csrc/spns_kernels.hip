@@ -1272,6 +1272,136 @@ __global__ __launch_bounds__(NT) void k_zplain(ZPlainArgs a)
 // src/HIT_LES.cpp:205-242 (R), :243-278 (constant power), :295-339 (AB2 /
 // kappa), :346-359 (projection); reductions :991-1003, :1137-1214.
 // ---------------------------------------------------------------------------
+// Step epilogue of one retained mode m = (k1, k2, k3) (src/HIT_LES.cpp:196-363):
+// R1 from the normalised convective term C (and, LES, the SGS stress S:
+// 11 21 22 31 32 33), viscosity, rotation, constant-power forcing; AB2 or the
+// kappa scheme; projection; the state and history written back; then the
+// shell-energy and enstrophy contributions of the new state.  With step
+// false only the reductions of the current state.
+struct EpiScalars {
+    double dt, nu, alpha, kap, k05;
+    bool first;
+};
+
+static __device__ __forceinline__ EpiScalars epi_scalars(const EpiArgs& a)
+{
+    EpiScalars q;
+    q.dt = a.scal->dt;
+    q.nu = a.scal->nu;
+    q.alpha = a.scal->alpha;
+    q.kap = a.scal->kappa;
+    q.k05 = 1.0 / (q.kap + 0.5);
+    q.first = a.scal->iter == 0;
+    return q;
+}
+
+static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScalars& q, int64_t m, int k1, int k2, int k3,
+                                         bool step, const cx<double>* C, const cx<double>* S, double* myh, double& om)
+{
+    const int ik2 = k1 * k1 + k2 * k2 + k3 * k3;
+    const double rad2 = (double)ik2;
+    const int K = shell_of(ik2);
+    cx<double> U[3], R0[3], R1[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) { U[c] = a.uk[c][m]; R0[c] = a.R[c][m]; }
+    if (step) {
+        const double dt = q.dt, kap = q.kap, k05 = q.k05, alpha = q.alpha;
+        const double ct = q.nu * rad2;
+        const bool mean = !(rad2 > 0.0);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            R1[c].x = -(mean ? 0.0 : C[c].x) - ct * U[c].x;
+            R1[c].y = -(mean ? 0.0 : C[c].y) - ct * U[c].y;
+        }
+        if (S) {
+            // S: 11, 21, 22, 31, 32, 33 (src/HIT_LES.cpp:179-191)
+            const double K1 = k1, K2 = k2, K3 = k3;
+            const double dxr = -K1 * S[0].y - K2 * S[1].y - K3 * S[3].y;
+            const double dxi = K1 * S[0].x + K2 * S[1].x + K3 * S[3].x;
+            const double dyr = -K1 * S[1].y - K2 * S[2].y - K3 * S[4].y;
+            const double dyi = K1 * S[1].x + K2 * S[2].x + K3 * S[4].x;
+            const double dzr = -K1 * S[3].y - K2 * S[4].y - K3 * S[5].y;
+            const double dzi = K1 * S[3].x + K2 * S[4].x + K3 * S[5].x;
+            R1[0].x -= dxr; R1[0].y -= dxi;
+            R1[1].x -= dyr; R1[1].y -= dyi;
+            R1[2].x -= dzr; R1[2].y -= dzi;
+        }
+        if (a.rot) {
+            const double ox = a.ox, oy = a.oy, oz = a.oz;
+            R1[0].x += 2 * (oz * U[1].x - oy * U[2].x); R1[0].y += 2 * (oz * U[1].y - oy * U[2].y);
+            R1[1].x += 2 * (ox * U[2].x - oz * U[0].x); R1[1].y += 2 * (ox * U[2].y - oz * U[0].y);
+            R1[2].x += 2 * (oy * U[0].x - ox * U[1].x); R1[2].y += 2 * (oy * U[0].y - ox * U[1].y);
+        }
+        if (a.force_on) {
+            if (a.kfmin <= K && K <= a.kfmax) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) { R1[c].x += alpha * U[c].x; R1[c].y += alpha * U[c].y; }
+            } else if (K == 0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) R1[c] = cx<double>{0.0, 0.0};
+            }
+        }
+        cx<double> N[3];
+        if (a.sa) {
+            // self-adaptive kappa scheme; the oldest level at the first
+            // step is the current field
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const cx<double> Up = a.ukprev[c][m];  // read unconditionally: a select of
+                const cx<double> Ua = q.first ? U[c] : Up;  // pointers would push U to scratch
+                a.ukprev[c][m] = U[c];
+                N[c].x = k05 * (2.0 * kap * U[c].x + dt * ((1.0 + kap) * R1[c].x - kap * R0[c].x) - (kap - 0.5) * Ua.x);
+                N[c].y = k05 * (2.0 * kap * U[c].y + dt * ((1.0 + kap) * R1[c].y - kap * R0[c].y) - (kap - 0.5) * Ua.y);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if (a.keepprev) a.ukprev[c][m] = U[c];  // the start-of-step state
+                N[c].x = U[c].x + dt * (1.5 * R1[c].x - 0.5 * R0[c].x);
+                N[c].y = U[c].y + dt * (1.5 * R1[c].y - 0.5 * R0[c].y);
+            }
+        }
+        {
+            const double K1 = k1, K2 = k2, K3 = k3;
+            const double ir2 = (rad2 > 0) ? 1.0 / rad2 : 0.0;
+            const double axr = (K1 * N[0].x + K2 * N[1].x + K3 * N[2].x) * ir2;
+            const double axi = (K1 * N[0].y + K2 * N[1].y + K3 * N[2].y) * ir2;
+            N[0].x -= axr * K1; N[1].x -= axr * K2; N[2].x -= axr * K3;
+            N[0].y -= axi * K1; N[1].y -= axi * K2; N[2].y -= axi * K3;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a.uk[c][m] = N[c];
+            a.R[c][m] = R1[c];
+            U[c] = N[c];
+        }
+    }
+    // reductions on the (new) state
+    const double ke = 0.5 * (U[0].x * U[0].x + U[0].y * U[0].y + U[1].x * U[1].x + U[1].y * U[1].y +
+                             U[2].x * U[2].x + U[2].y * U[2].y);
+    const bool conj = (k3 != 0) && !((a.Mz % 2 == 0) && (k3 == a.Mz / 2));
+    atomicAdd(&myh[K], conj ? 2 * ke : ke);
+    const double en = 2 * rad2 * ke;
+    om += conj ? 2 * en : en;
+}
+
+// Per-wave shell histograms [NW][nbins] summed in wave order into the
+// block's partial, and the block's enstrophy partial.
+template <int NT>
+__device__ __forceinline__ void epi_flush(const EpiArgs& a, const double* hist, double om, double* red)
+{
+    constexpr int NW = NT / 64;
+    const int nb = a.nbins;
+    __syncthreads();
+    for (int K = threadIdx.x; K < nb; K += NT) {
+        double s = 0.0;
+        for (int w = 0; w < NW; w++) s += hist[w * nb + K];
+        a.hist_partial[(int64_t)blockIdx.x * nb + K] = s;
+    }
+    om = block_sum<NT>(om, red);
+    if (threadIdx.x == 0) a.om_partial[blockIdx.x] = om;
+}
+
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
 {
@@ -1282,13 +1412,11 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
     for (int i = threadIdx.x; i < NW * nb; i += NT) hist[i] = 0.0;
     __syncthreads();
     double* myh = hist + (threadIdx.x / 64) * nb;
-    const double dt = a.scal->dt, nu = a.scal->nu, alpha = a.scal->alpha;
-    const double kap = a.scal->kappa;
-    const double k05 = 1.0 / (kap + 0.5);
-    const bool first = a.scal->iter == 0;
+    const EpiScalars q = epi_scalars(a);
     double om = 0.0;
     const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3, nk3p = a.nk3p;
     const double inv_nf = 1.0 / a.nf;
+    const bool step = a.mode == EPI_STEP;
     // one wave per (i2, i1) row of the compact box, lanes along i3: no
     // per-mode index division, coalesced row reads
     const int lane = threadIdx.x & 63;
@@ -1299,108 +1427,134 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
         const int64_t m = (int64_t)row * nk3p + i3;
         const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2), k3 = i3;
         if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) continue;  // src/HIT_LES.cpp:867-871
-        const int ik2 = k1 * k1 + k2 * k2 + k3 * k3;
-        const double rad2 = (double)ik2;
-        const int K = shell_of(ik2);
-        cx<double> U[3], R0[3], R1[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) { U[c] = a.uk[c][m]; R0[c] = a.R[c][m]; }
-        if (a.mode == EPI_STEP) {
-            const double ct = nu * rad2;
-            const bool mean = !(rad2 > 0.0);
+        cx<double> C[3], S[6];
+        if (step) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const cx<T> cv = ((const cx<T>*)a.conv[c])[m];
-                const double cvx = (double)cv.x * inv_nf, cvy = (double)cv.y * inv_nf;  // Fourier_Normalization
-                R1[c].x = -(mean ? 0.0 : cvx) - ct * U[c].x;
-                R1[c].y = -(mean ? 0.0 : cvy) - ct * U[c].y;
+                C[c] = cx<double>{(double)cv.x * inv_nf, (double)cv.y * inv_nf};  // Fourier_Normalization
             }
             if (a.les) {
-                cx<double> S[6];
 #pragma unroll
                 for (int s = 0; s < 6; s++) {
                     const cx<T> v = ((const cx<T>*)a.conv[3 + s])[m];
-                    S[s].x = (double)v.x * inv_nf;
-                    S[s].y = (double)v.y * inv_nf;
+                    S[s] = cx<double>{(double)v.x * inv_nf, (double)v.y * inv_nf};
                 }
-                // S: 11, 21, 22, 31, 32, 33 (src/HIT_LES.cpp:179-191)
-                const double K1 = k1, K2 = k2, K3 = k3;
-                const double dxr = -K1 * S[0].y - K2 * S[1].y - K3 * S[3].y;
-                const double dxi = K1 * S[0].x + K2 * S[1].x + K3 * S[3].x;
-                const double dyr = -K1 * S[1].y - K2 * S[2].y - K3 * S[4].y;
-                const double dyi = K1 * S[1].x + K2 * S[2].x + K3 * S[4].x;
-                const double dzr = -K1 * S[3].y - K2 * S[4].y - K3 * S[5].y;
-                const double dzi = K1 * S[3].x + K2 * S[4].x + K3 * S[5].x;
-                R1[0].x -= dxr; R1[0].y -= dxi;
-                R1[1].x -= dyr; R1[1].y -= dyi;
-                R1[2].x -= dzr; R1[2].y -= dzi;
-            }
-            if (a.rot) {
-                const double ox = a.ox, oy = a.oy, oz = a.oz;
-                R1[0].x += 2 * (oz * U[1].x - oy * U[2].x); R1[0].y += 2 * (oz * U[1].y - oy * U[2].y);
-                R1[1].x += 2 * (ox * U[2].x - oz * U[0].x); R1[1].y += 2 * (ox * U[2].y - oz * U[0].y);
-                R1[2].x += 2 * (oy * U[0].x - ox * U[1].x); R1[2].y += 2 * (oy * U[0].y - ox * U[1].y);
-            }
-            if (a.force_on) {
-                if (a.kfmin <= K && K <= a.kfmax) {
-#pragma unroll
-                    for (int c = 0; c < 3; c++) { R1[c].x += alpha * U[c].x; R1[c].y += alpha * U[c].y; }
-                } else if (K == 0) {
-#pragma unroll
-                    for (int c = 0; c < 3; c++) R1[c] = cx<double>{0.0, 0.0};
-                }
-            }
-            cx<double> N[3];
-            if (a.sa) {
-                // self-adaptive kappa scheme; the oldest level at the first
-                // step is the current field
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const cx<double> Up = a.ukprev[c][m];  // read unconditionally: a select of
-                    const cx<double> Ua = first ? U[c] : Up;  // pointers would push U to scratch
-                    a.ukprev[c][m] = U[c];
-                    N[c].x = k05 * (2.0 * kap * U[c].x + dt * ((1.0 + kap) * R1[c].x - kap * R0[c].x) - (kap - 0.5) * Ua.x);
-                    N[c].y = k05 * (2.0 * kap * U[c].y + dt * ((1.0 + kap) * R1[c].y - kap * R0[c].y) - (kap - 0.5) * Ua.y);
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    if (a.keepprev) a.ukprev[c][m] = U[c];  // the start-of-step state
-                    N[c].x = U[c].x + dt * (1.5 * R1[c].x - 0.5 * R0[c].x);
-                    N[c].y = U[c].y + dt * (1.5 * R1[c].y - 0.5 * R0[c].y);
-                }
-            }
-            {
-                const double K1 = k1, K2 = k2, K3 = k3;
-                const double ir2 = (rad2 > 0) ? 1.0 / rad2 : 0.0;
-                const double axr = (K1 * N[0].x + K2 * N[1].x + K3 * N[2].x) * ir2;
-                const double axi = (K1 * N[0].y + K2 * N[1].y + K3 * N[2].y) * ir2;
-                N[0].x -= axr * K1; N[1].x -= axr * K2; N[2].x -= axr * K3;
-                N[0].y -= axi * K1; N[1].y -= axi * K2; N[2].y -= axi * K3;
-            }
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                a.uk[c][m] = N[c];
-                a.R[c][m] = R1[c];
-                U[c] = N[c];
             }
         }
-        // reductions on the (new) state
-        const double ke = 0.5 * (U[0].x * U[0].x + U[0].y * U[0].y + U[1].x * U[1].x + U[1].y * U[1].y +
-                                 U[2].x * U[2].x + U[2].y * U[2].y);
-        const bool conj = (k3 != 0) && !((a.Mz % 2 == 0) && (k3 == a.Mz / 2));
-        atomicAdd(&myh[K], conj ? 2 * ke : ke);
-        const double en = 2 * rad2 * ke;
-        om += conj ? 2 * en : en;
+        epi_mode(a, q, m, k1, k2, k3, step, C, (step && a.les) ? S : nullptr, myh, om);
     }
+    epi_flush<NT>(a, hist, om, red);
+}
+
+// ---------------------------------------------------------------------------
+// Forward y pass of the three DNS convective fields fused with the step
+// epilogue.  A persistent block walks items (k1 index, tile of TB k3
+// columns); per item the three components run one after the other through
+// the staged strided transform (input tile copied asynchronously, the next
+// component's copy issued after stage 0), components 0 and 1 keep their
+// retained outputs in registers, and after component 2 every retained mode
+// of the item goes through epi_mode.  The convective spectra never reach
+// HBM.  Shell histograms are per wave, as in k_epilogue.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int TB>
+__host__ __device__ constexpr size_t yepi_static_lds()
+{
+    return sizeof(cx<T>) * ((size_t)2 * L * TB + fft_twiddle_span<L, sizeof(T) == 8 ? 1 : 2>());
+}
+
+template <typename T, int L, int TB, int NT>
+__global__ __launch_bounds__(NT) void k_yepi(StridedArgs a, EpiArgs e)
+{
+    constexpr int SPR = TB * (int)sizeof(cx<T>) / 16;  // 16-byte copy slots per staged row
+    constexpr int EPS = 16 / (int)sizeof(cx<T>);
+    constexpr int NW = NT / 64;
+    constexpr int TWM = sizeof(T) == 8 ? 1 : 2;
+    constexpr int TWN = fft_twiddle_span<L, TWM>();
+    constexpr int NOUTMAX = 2 * (L / 3) + 1;  // the launcher checks nout against it
+    constexpr int Q = (NOUTMAX * TB + NT - 1) / NT;
+    static_assert(L * TB * sizeof(cx<T>) >= NT * sizeof(double), "reduction scratch aliases fbuf");
+    __shared__ __align__(16) cx<T> fbuf[L * TB];
+    __shared__ __align__(16) cx<T> stg[L * TB];
+    __shared__ __align__(16) cx<T> twl[TWN];
+    extern __shared__ double hist[];  // [NW][nbins]
+    const int t = threadIdx.x;
+    const int nb = e.nbins;
+    for (int i = t; i < NW * nb; i += NT) hist[i] = 0.0;
+    for (int i = t; i < TWN; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
     __syncthreads();
-    for (int K = threadIdx.x; K < nb; K += NT) {
-        double s = 0.0;
-        for (int w = 0; w < NW; w++) s += hist[w * nb + K];
-        a.hist_partial[(int64_t)blockIdx.x * nb + K] = s;
+    double* myh = hist + (t >> 6) * nb;
+    const EpiScalars q = epi_scalars(e);
+    double om = 0.0;
+    const double inv_nf = 1.0 / e.nf;
+    const int nout = a.nout, hout = nout / 2;
+    const int ntiles = (a.B + TB - 1) / TB;
+    const int64_t nitems = (int64_t)a.A * ntiles;
+    const int G = gridDim.x;  // a multiple of 8
+    const int64_t lin = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+    const uint32_t stg_base = lds_addr(stg);
+    const int nslot = a.nin * SPR;
+    auto issue = [&](int64_t it, int c) {
+        const int64_t ai = it / ntiles;
+        const int b0 = (int)(it - ai * ntiles) * TB;
+        const cx<T>* in = (const cx<T>*)a.job[c].in + ai * a.in_a_stride + b0;
+        for (int m = wave; m * 64 < nslot; m += NW) {
+            const int s = m * 64 + lane;
+            const int row = s / SPR, part = s - row * SPR;
+            if (s < nslot) dma16(in + (int64_t)row * a.in_row_stride + part * EPS, stg_base + (uint32_t)m * 1024u);
+        }
+    };
+    auto load = [&](int b, int p) -> cx<T> { return stg[p * TB + b]; };
+    auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * TB + b] = v; };
+
+    int64_t it = lin;
+    if (it < nitems) issue(it, 0);
+    dma_wait();
+    lds_barrier();
+    for (; it < nitems; it += G) {
+        const int64_t ai = it / ntiles;
+        const int b0 = (int)(it - ai * ntiles) * TB;
+        const int i1 = (int)ai;
+        const int k1 = wrap_k(i1, e.nk1);
+        cx<T> keep[2][Q];
+        auto comp = [&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            auto prefetch = [&]() {
+                if (c < 2) issue(it, c + 1);
+                else if (it + G < nitems) issue(it + G, 0);
+            };
+            FftS<T, L, TB, NT, TB, -1, false, TWM>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
+            lds_barrier();
+            dma_wait();  // the next component's tile (this wave's copies) has landed
+#pragma unroll
+            for (int qq = 0; qq < Q; qq++) {
+                const int idx = t + qq * NT;
+                if (idx < nout * TB) {
+                    const int i = idx / TB, b = idx - i * TB;
+                    const int p = (i <= hout) ? i : L - (nout - i);
+                    if constexpr (c < 2) {
+                        keep[c][qq] = fbuf[p * TB + b];
+                    } else {
+                        const int k2 = wrap_k(i, e.nk2), k3 = b0 + b;
+                        if (k3 < a.B && !(e.notcubic && ((k1 % e.Lx) || (k2 % e.Ly) || (k3 % e.Lz)))) {  // src/HIT_LES.cpp:867-871
+                            const cx<T> v2 = fbuf[p * TB + b];
+                            const cx<double> C[3] = {
+                                cx<double>{(double)keep[0][qq].x * inv_nf, (double)keep[0][qq].y * inv_nf},
+                                cx<double>{(double)keep[1][qq].x * inv_nf, (double)keep[1][qq].y * inv_nf},
+                                cx<double>{(double)v2.x * inv_nf, (double)v2.y * inv_nf}};  // Fourier_Normalization
+                            const int64_t m = ((int64_t)i * e.nk1 + i1) * e.nk3p + k3;
+                            epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om);
+                        }
+                    }
+                }
+            }
+            lds_barrier();  // every wave's copies are in; fbuf is free again
+        };
+        static_for<3>(comp);
     }
-    om = block_sum<NT>(om, red);
-    if (threadIdx.x == 0) a.om_partial[blockIdx.x] = om;
+    epi_flush<NT>(e, hist, om, (double*)fbuf);
 }
 
 // ---------------------------------------------------------------------------
@@ -2006,6 +2160,71 @@ hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* 
     const int n = 5 * a.nbins;
     hipLaunchKernelGGL(k_inv_sum, dim3((n + 255) / 256), dim3(256), 0, st, a.partial, grid, n, out);
     return hipGetLastError();
+}
+
+// Fused forward y pass + step epilogue (DNS): grid for A x tiles items with
+// nbins shell bins, or 0 where the kernel does not apply (staging or LDS
+// does not fit, or more retained rows than the 3/2 padding leaves).
+template <typename T, int L>
+static int yepi_grid_L(int nbins, int nout, int64_t items, int max_grid)
+{
+    constexpr int TB = strided_tb<T, L>();
+    if constexpr (TB <= 0 || (TB * sizeof(cx<T>)) % 16 != 0) {
+        return 0;
+    } else {
+        constexpr int NT = sstage_nt<L, TB>();
+        const size_t shm = sizeof(double) * (NT / 64) * (size_t)nbins;
+        if (yepi_static_lds<T, L, TB>() + shm > 160 * 1024 - 512 || nout > 2 * (L / 3) + 1) return 0;
+        // a variant that spills registers (the 1024-thread tiles) is slower
+        // than the two separate kernels
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_yepi<T, L, TB, NT>)) != hipSuccess || fa.localSizeBytes > 0)
+            return 0;
+        int n = 0, dev = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_yepi<T, L, TB, NT>, NT, shm) != hipSuccess || n < 1) return 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        int64_t g = (int64_t)ncu * n;
+        if (items < g) g = items;
+        g = (g + 7) / 8 * 8;
+        if (g > max_grid) g = max_grid / 8 * 8;
+        return (int)g;
+    }
+}
+
+int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid)
+{
+#define X(LL) \
+    if (L == LL) return prec == PREC_FP64 ? yepi_grid_L<double, LL>(nbins, nout, items, max_grid) : yepi_grid_L<float, LL>(nbins, nout, items, max_grid);
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return 0;
+}
+
+template <typename T, int L>
+static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st)
+{
+    constexpr int TB = strided_tb<T, L>();
+    if constexpr (TB <= 0 || (TB * sizeof(cx<T>)) % 16 != 0) {
+        return hipErrorNotSupported;
+    } else {
+        constexpr int NT = sstage_nt<L, TB>();
+        StridedArgs b = a;
+        b.A = A;
+        if (!a.padded_ok || a.nin != L || a.njobs != 3 || a.nout > 2 * (L / 3) + 1 || grid < 8 || grid % 8) return hipErrorInvalidValue;
+        const size_t shm = sizeof(double) * (NT / 64) * (size_t)e.nbins;
+        hipLaunchKernelGGL((k_yepi<T, L, TB, NT>), dim3(grid), dim3(NT), shm, st, b, e);
+        return hipGetLastError();
+    }
+}
+
+hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st)
+{
+#define X(LL) \
+    if (L == LL) return prec == PREC_FP64 ? launch_yepi_L<double, LL>(a, A, e, grid, st) : launch_yepi_L<float, LL>(a, A, e, grid, st);
+    SPNS_FOR_EACH_L(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st)

@@ -168,3 +168,32 @@ def test_velocity_step_start(kw):
             h.velocity_physical_step_start()
     g.close()
     h.close()
+
+
+@pytest.mark.parametrize("kw", [dict(Nx=32, nu=1e-3, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2),
+                                dict(Nx=32, nu=1e-3, C_At=0.5, isSelfAdaptive=True),
+                                dict(Nx=16, Ny=16, Nz=512, nu=2e-3, C_At=0.5, omega=(0.0, 0.0, 3.0))])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_epilogue_matches_split(monkeypatch, kw, precision):
+    """The DNS step's forward y pass fused with the epilogue against the two
+    separate kernels (SPNS_EPI=split): same state, history and scalars up to
+    the summation order of the shell sums (which feeds the forcing and dt)."""
+    out = {}
+    for mode in ("split", "fused"):
+        if mode == "split":
+            monkeypatch.setenv("SPNS_EPI", "split")
+        else:
+            monkeypatch.delenv("SPNS_EPI", raising=False)
+        g = _solver(precision=precision, **kw)
+        g.init_random(seed=9, k0=4.0)
+        g.finish_init(False)
+        g.step(6)
+        out[mode] = (g.get_state(), g.get_rhist(), g.spectrum(), g.scalars())
+        g.close()
+    s, f = out["split"], out["fused"]
+    tol = 1e-12 if precision == 0 else 1e-6
+    assert rel(f[0], s[0]) <= tol
+    assert rel(f[1], s[1]) <= tol
+    assert rel(f[2], s[2]) <= tol
+    for k in ("time", "dt", "ek", "om", "skewness"):
+        assert abs(f[3][k] - s[3][k]) <= tol * max(1.0, abs(s[3][k])), k
