@@ -142,7 +142,11 @@ __host__ __device__ constexpr bool sstage_fits()
     return sstage_lds_bytes<TIN, T, L, TB, DIR>() <= 160 * 1024 - 512 && (TB * sizeof(cx<TIN>)) % 16 == 0;
 }
 
-template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR>
+// STDP: inverse pass at the standard 3/2 padding (nin = 2 (L/3) + 1 stored
+// rows) with L/R0 = 64 and one stage-0 butterfly per thread; the zone of
+// each stage-0 input (stored low rows, stored high rows, zero padding) is
+// then fixed at compile time.
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
 __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
 {
     constexpr int SROWS = sstage_rows<L, DIR>();
@@ -205,6 +209,31 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
             }
             return cx<T>{(T)v.x, (T)v.y};
         };
+        auto load_std = [&](int b, int j, auto rc) -> cx<T> {
+            constexpr int r = decltype(rc)::value;
+            constexpr int p0 = 64 * r;
+            constexpr int H = L / 3, NIN = 2 * H + 1;
+            constexpr bool LO = p0 + 63 <= H, HI = p0 >= L - H, ZERO = p0 > H && p0 + 63 < L - H;
+            if constexpr (ZERO) {
+                return cx<T>{(T)0, (T)0};
+            } else {
+                const int p = p0 + j;
+                bool lo = LO, hi = HI;
+                if constexpr (!LO && !HI) {
+                    lo = p <= H;
+                    hi = p >= L - H;
+                    if (!(lo || hi)) return cx<T>{(T)0, (T)0};
+                }
+                const int i = lo ? p : p - (L - NIN);
+                const cx<TIN> v = stg[i * TB + b];
+                if (deriv) {
+                    // i k u, formed in the input precision (src/HIT_LES.cpp:37)
+                    const TIN kk = (TIN)(lo ? p : p - L);
+                    return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
+                }
+                return cx<T>{(T)v.x, (T)v.y};
+            }
+        };
         auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * TB + b] = v; };
         auto prefetch = [&]() {
             if (nxt < nitems) issue(nxt);
@@ -220,17 +249,24 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
                 }
             }
         };
+        auto fft = [&](auto lead, auto& hook) {
+            constexpr bool LEAD = decltype(lead)::value;
+            if constexpr (STDP)
+                FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<LEAD>(fbuf, twl, load_std, store_lds, hook, t);
+            else
+                FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<LEAD>(fbuf, twl, load, store_lds, hook, t);
+        };
         if (dual) {
             // plain transform first, from the staged tile; the tile stays
             // staged (no prefetch yet) for the i k transform below
             auto none = [] {};
-            FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<false>(fbuf, twl, load, store_lds, none, t);
+            fft(std::false_type{}, none);
             lds_barrier();
             write_tile(a.job[jb].out);
             deriv = 1;
-            FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<true>(fbuf, twl, load, store_lds, prefetch, t);
+            fft(std::true_type{}, prefetch);
         } else {
-            FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
+            fft(std::false_type{}, prefetch);
         }
         lds_barrier();
         dma_wait();  // the next item's tile (this wave's copies) has landed
@@ -1767,6 +1803,8 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
     if constexpr (sstage_fits<TIN, T, L, TB, DIR>()) {
         if (a.padded_ok && a.nin <= sstage_rows<L, DIR>()) {
             constexpr int NT = sstage_nt<L, TB>();
+            constexpr bool STDP_OK = DIR > 0 && L / Plan<L>::r(0) == 64 && TB * 64 <= NT;
+            const bool stdp = STDP_OK && a.nin == 2 * (L / 3) + 1;
             static const int occ = [] {
                 int n = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sstage<TIN, TOUT, T, L, TB, NT, DIR>, NT, 0) != hipSuccess) n = 1;
@@ -1780,6 +1818,12 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
             int64_t g = (int64_t)ncu * occ;
             if (items < g) g = items;
             g = (g + 7) / 8 * 8;
+            if constexpr (STDP_OK) {
+                if (stdp) {
+                    hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR, true>), dim3((unsigned)g), dim3(NT), 0, st, b);
+                    return hipGetLastError();
+                }
+            }
             hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR>), dim3((unsigned)g), dim3(NT), 0, st, b);
             return hipGetLastError();
         }
