@@ -143,8 +143,8 @@ __host__ __device__ constexpr bool sstage_fits()
 }
 
 // STDP: inverse pass at the standard 3/2 padding (nin = 2 (L/3) + 1 stored
-// rows) with L/R0 = 64 and one stage-0 butterfly per thread; the zone of
-// each stage-0 input (stored low rows, stored high rows, zero padding) is
+// rows) with L/R0 = 64 or 128 and one stage-0 butterfly per thread; the zone
+// of each stage-0 input (stored low rows, stored high rows, zero padding) is
 // then fixed at compile time.
 template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
 __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
@@ -211,9 +211,10 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
         };
         auto load_std = [&](int b, int j, auto rc) -> cx<T> {
             constexpr int r = decltype(rc)::value;
-            constexpr int p0 = 64 * r;
+            constexpr int LR0 = L / Plan<L>::r(0);
+            constexpr int p0 = LR0 * r;
             constexpr int H = L / 3, NIN = 2 * H + 1;
-            constexpr bool LO = p0 + 63 <= H, HI = p0 >= L - H, ZERO = p0 > H && p0 + 63 < L - H;
+            constexpr bool LO = p0 + LR0 - 1 <= H, HI = p0 >= L - H, ZERO = p0 > H && p0 + LR0 - 1 < L - H;
             if constexpr (ZERO) {
                 return cx<T>{(T)0, (T)0};
             } else {
@@ -910,8 +911,9 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
     // inverse pair b (0..5): logical fields (2b, 2b+1) of u v w ux uy uz vx vy vz wx wy wz
     constexpr uint32_t SRC_A = 0x814620u, SRC_B = 0x257031u, DZ_A = 0x10u, DZ_B = 0x24u;
     // compile-time stage-0 loader: needs the compile-time k3 count, L/R0 = 64
-    // and at most one stage-0 butterfly per thread
-    constexpr bool FAST = NKC > 0 && L / Plan<L>::r(0) == 64 && NB * 64 <= NT;
+    // or 128 and at most one stage-0 butterfly per thread
+    constexpr int LR0 = L / Plan<L>::r(0);
+    constexpr bool FAST = NKC > 0 && (LR0 == 64 || LR0 == 128) && NB * LR0 <= NT;
     constexpr bool S0COL = sizeof(T) == 4;  // column-major stage 0 in FP32 (its writes conflict in FP64)
 
     int64_t pen = blockIdx.x;
@@ -947,27 +949,27 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
             const T by = (p == 0) ? (T)0 : s * B.y;
             return cx<T>{A.x - by, ay + B.x};
         };
-        // Fast loader.  With L/R0 = 64 and one stage-0 butterfly per thread,
-        // input p = 64 r + j with j < 64, so the zone of p (direct k3 = p,
-        // mirrored k3 = L - p, or zero padding) is fixed for each r at compile
-        // time.  The thread's column is the wave index (a scalar) when stage 0
-        // is column-major, else t mod 8.  Columns pair each velocity component
+        // Fast loader.  With L/R0 = LR0 (64 or 128) and one stage-0 butterfly
+        // per thread, input p = LR0 r + j with j < LR0, so the zone of p
+        // (direct k3 = p, mirrored k3 = L - p, or zero padding) is fixed for
+        // each r at compile time.  The thread's column is the wave index over
+        // LR0 / 64 (a scalar) when stage 0 is column-major, else t mod 8.  Columns pair each velocity component
         // with its own z derivative, (f, i k f) -> (1 - k) f direct and
         // (1 + k) conj f mirrored (one read, no derivative product), then
         // (fx, fy) of u, v, w.
-        const int col = S0COL ? wave : (tl & (NB - 1));
+        const int col = S0COL ? wave / (LR0 / 64) : (tl & (NB - 1));
         const cx<T>* pa = stg + col * SP;        // f (columns 0-2) or fx (3-5)
         const cx<T>* pb = stg + (col + 3) * SP;  // fy (columns 3-5)
         auto load_fast = [&](int, int j, auto rc) -> cx<T> {
             constexpr int r = decltype(rc)::value;
-            constexpr int p0 = 64 * r;
+            constexpr int p0 = LR0 * r;
             if (col >= 6) {
                 if (!havep) return cx<T>{(T)0, (T)0};
                 const cx<T> v = fbuf[(p0 + j) * LD + col];
                 return cx<T>{v.x, -v.y};  // forward = conj(inverse(conj x))
             }
             if (!have) return cx<T>{(T)0, (T)0};
-            constexpr bool LO = p0 + 63 < NKC, HI = p0 > L - NKC, ZERO = p0 >= NKC && p0 + 63 <= L - NKC;
+            constexpr bool LO = p0 + LR0 - 1 < NKC, HI = p0 > L - NKC, ZERO = p0 >= NKC && p0 + LR0 - 1 <= L - NKC;
             if constexpr (ZERO) {
                 return cx<T>{(T)0, (T)0};
             } else {
@@ -1803,7 +1805,8 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
     if constexpr (sstage_fits<TIN, T, L, TB, DIR>()) {
         if (a.padded_ok && a.nin <= sstage_rows<L, DIR>()) {
             constexpr int NT = sstage_nt<L, TB>();
-            constexpr bool STDP_OK = DIR > 0 && L / Plan<L>::r(0) == 64 && TB * 64 <= NT;
+            constexpr int LR0 = L / Plan<L>::r(0);
+            constexpr bool STDP_OK = DIR > 0 && (LR0 == 64 || LR0 == 128) && TB * LR0 <= NT;
             const bool stdp = STDP_OK && a.nin == 2 * (L / 3) + 1;
             static const int occ = [] {
                 int n = 0;
