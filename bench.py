@@ -97,11 +97,19 @@ def workload_nu(N):
     return 8e-5 if N >= 1024 else 2e-4
 
 
-def run_form(N, prec, steps, warmup, seed, prof_steps):
+LES_NU, LES_CS, LES_K0 = 1e-4, 0.17, 6.0  # the Smagorinsky LES workload (--les)
+
+
+def run_form(N, prec, steps, warmup, seed, prof_steps, les=False):
     import spns_amd
-    s = spns_amd.Solver(Nx=N, nu=workload_nu(N), C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2,
-                        precision=prec, use_graph=True)
-    s.init_random(seed=seed, k0=4.0, urms=1.0)
+    if les:
+        # decaying Smagorinsky LES, Delta = 2 pi / N, CFL step
+        s = spns_amd.Solver(Nx=N, nu=LES_NU, C_Smag=LES_CS, C_At=0.5, delta=2.0 * math.pi / N, precision=prec,
+                            use_graph=True)
+    else:
+        s = spns_amd.Solver(Nx=N, nu=workload_nu(N), C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2,
+                            precision=prec, use_graph=True)
+    s.init_random(seed=seed, k0=LES_K0 if les else 4.0, urms=1.0)
     s.finish_init(False)
     s.step(warmup)
     sync_all(s)
@@ -138,13 +146,17 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(n_cpu, threads, steps):
-    """Median wall time of `steps` steps of the same forced-DNS setup at
-    N=n_cpu on the host (the FP64 C oracle, 1 process x OpenMP threads)."""
+def cpu_baseline(n_cpu, threads, steps, les=False):
+    """Median wall time of `steps` steps of the same forced-DNS (or LES)
+    setup at N=n_cpu on the host (the FP64 C oracle, 1 process x OpenMP
+    threads)."""
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     from oracle.oracle import Oracle, lib
     import numpy as np
-    o = Oracle(n_cpu, nu=2e-4, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2)
+    if les:
+        o = Oracle(n_cpu, nu=LES_NU, C_Smag=LES_CS, C_At=0.5, delta=2.0 * math.pi / n_cpu)
+    else:
+        o = Oracle(n_cpu, nu=2e-4, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2)
     rng = np.random.default_rng(0)
     st = (rng.standard_normal((3, o.nmodes)) + 1j * rng.standard_normal((3, o.nmodes))) * 1e-3
     o.set_state(st)
@@ -222,6 +234,9 @@ def main():
                     help="requests of the serving mix to run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--les", action="store_true",
+                    help="the Smagorinsky LES workload (nu=1e-4, C_s=0.17, Delta=2pi/N, k0=6) instead of the "
+                         "forced DNS; use with --N 256")
     ap.add_argument("--copy-peak", action="store_true", help="only print the measured copy bandwidth and exit")
     ap.add_argument("--no-copy-peak", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -256,7 +271,7 @@ def main():
     for form in forms:
         prec = 0 if form == "fp64" else 1
         print("bench: %s N=%d (M=%d) warmup=%d steps=%d" % (form, N, M, a.warmup, a.steps), file=sys.stderr, flush=True)
-        res[form] = run_form(N, prec, a.steps, a.warmup, a.seed, a.prof_steps)
+        res[form] = run_form(N, prec, a.steps, a.warmup, a.seed, a.prof_steps, les=a.les)
         r = res[form]
         print("bench: %s %.3f ms/step (events %.3f ms over %d steps)" % (form, 1e3 * r["wall_s"] / a.steps,
               r["event_ms"] / a.steps, a.steps), file=sys.stderr, flush=True)
@@ -274,7 +289,7 @@ def main():
         return bool(p) and p.get("epilogue", 1.0) < 0.02 and p.get("fwd_y", 0.0) > 0.0
 
     # dominant kernel roofline from per-pass HIP-event timings
-    pb = pass_bytes(N, prec_main, fused_epilogue=fused(r))
+    pb = pass_bytes(N, prec_main, les=a.les, fused_epilogue=fused(r))
     passes = r["passes"]
     roof = None
     if passes:
@@ -293,8 +308,8 @@ def main():
             "steps_per_s": sp,
             "ms_per_step": 1e3 * rr["wall_s"] / a.steps,
             "grid_points_steps_per_s": sp * M ** 3,
-            "hbm_pct_model": 100.0 * model_bytes(N, prec) * sp / PEAK_HBM,
-            "bytes_per_step_algorithmic": sum(pass_bytes(N, prec, fused_epilogue=fused(rr)).values()),
+            "hbm_pct_model": 100.0 * model_bytes(N, prec, les=a.les) * sp / PEAK_HBM,
+            "bytes_per_step_algorithmic": sum(pass_bytes(N, prec, les=a.les, fused_epilogue=fused(rr)).values()),
             "fwd_y_fused_with_epilogue": fused(rr),
             "pass_ms": rr["passes"],
             "ek_total": rr["scalars"]["ek"],
@@ -325,14 +340,14 @@ def main():
     if not a.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         print("bench: cpu baseline, oracle %d steps at N=%d" % (a.cpu_steps, a.cpu_N), file=sys.stderr, flush=True)
-        dt, ts, used = cpu_baseline(a.cpu_N, threads, a.cpu_steps)
+        dt, ts, used = cpu_baseline(a.cpu_N, threads, a.cpu_steps, les=a.les)
         Mc = dims(a.cpu_N)[0]
         scale = (M ** 3 * math.log2(M ** 3)) / (Mc ** 3 * math.log2(Mc ** 3))
         cpu = {"value": 1.0 / (dt * scale), "unit": "steps/s", "cores": used, "kind": "port",
-               "sample": "%d steps of the same forced-DNS setup at N=%d (FFT grid %d^3), median %.2f s/step "
+               "sample": "%d steps of the same %s setup at N=%d (FFT grid %d^3), median %.2f s/step "
                          "(%s), 1 process x %d OpenMP threads on %s; value scaled to N=%d by the P*log2(P) "
-                         "ratio %.2f" % (a.cpu_steps, a.cpu_N, Mc, dt, ", ".join("%.2f" % t for t in ts), used,
-                                         cpu_model(), N, scale),
+                         "ratio %.2f" % (a.cpu_steps, "LES" if a.les else "forced-DNS", a.cpu_N, Mc, dt,
+                                         ", ".join("%.2f" % t for t in ts), used, cpu_model(), N, scale),
                "measured_steps_per_s_at_sample": 1.0 / dt}
 
     line = {
@@ -348,8 +363,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32/f64 split" if main_form == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": "forced DNS %d^3 retained modes (FFT grid %d^3), nu=%g, constant power 0.1 "
-                               "on 1<=K<=2, CFL C_At=0.5, k0=4 u_rms=1 seed %d" % (N, M, workload_nu(N), a.seed),
+        "config": {"workload": ("Smagorinsky LES %d^3 retained modes (FFT grid %d^3), nu=%g, C_s=%g, Delta=2pi/%d, "
+                                "CFL C_At=0.5, k0=%g u_rms=1 seed %d" % (N, M, LES_NU, LES_CS, N, LES_K0, a.seed))
+                   if a.les else
+                   ("forced DNS %d^3 retained modes (FFT grid %d^3), nu=%g, constant power 0.1 "
+                    "on 1<=K<=2, CFL C_At=0.5, k0=4 u_rms=1 seed %d" % (N, M, workload_nu(N), a.seed)),
                    "N": N, "M": M, "form": main_form, "parallelism": "1 GPU"},
         "forms": forms_out,
         "fp32_split_speedup": (forms_out["fp32"]["steps_per_s"] / forms_out["fp64"]["steps_per_s"]
