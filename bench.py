@@ -178,6 +178,7 @@ def run_ensemble(n, seed):
     reqs = serve.draw_requests(n, rate=20.0, seed=seed, steps=(100, 2000))
     srv = serve.Server(tuner=serve.Tuner())
     stats, done = srv.run(reqs)
+    stats["policy"] = {"inflight_work": srv.inflight_work, "min_inflight": srv.min_inflight}
     steps = sum(r.steps for r in reqs)
     out = dict(stats)
     out.update({"arrival_rate_per_s": 20.0, "seed": seed, "total_steps": steps,

