@@ -89,7 +89,7 @@ def test_step_parity(name, graph):
         compare(g, o, "%s step+%d" % (name, chunk))
 
 
-def test_config1_full_200_steps():
+def test_decaying_dns_32_full_200_steps():
     """32^3 decaying DNS, nu=0.01, fixed dt=1e-3, 200 steps (seed 1, k0=4)."""
     g, o = pair(seed=1, k0=4.0, Nx=32, nu=0.01, fixed_dt=1e-3)
     g.finish_init(False)
@@ -100,7 +100,7 @@ def test_config1_full_200_steps():
         o.step(1)
         ek_hist_g.append(g.scalars()["ek"])
         ek_hist_o.append(o.energy())
-    compare(g, o, "config1 200 steps")
+    compare(g, o, "32^3 decaying DNS, 200 steps")
     assert rel(ek_hist_g, ek_hist_o) <= TOL
     o.antitransform()  # physical field of the current state on both sides
     assert rel(g.velocity_physical(), o.velocity_physical()) <= TOL
