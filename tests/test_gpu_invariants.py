@@ -22,6 +22,7 @@ CASES = {
     "rect": dict(Nx=32, Ny=16, Nz=24, nu=2e-3, C_At=0.5),
     "sa32": dict(Nx=32, nu=1e-3, C_At=0.5, isSelfAdaptive=True),
     "z768_full_k3": dict(Nx=16, Ny=16, Nz=512, nu=2e-3, C_At=0.5, force_eps=0.1),
+    "noncubic_112": dict(Nx=32, Ny=32, Nz=16, Mx=48, My=48, Mz=24, Lx=2, Ly=2, Lz=1, isNotCubic=True, nu=1e-3, C_At=0.5),
 }
 NAMES = ("QG", "QS", "RG", "RS", "V2")
 
