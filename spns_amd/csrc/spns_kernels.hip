@@ -2212,6 +2212,15 @@ hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* 
 // Fused forward y pass + step epilogue (DNS): grid for A x tiles items with
 // nbins shell bins, or 0 where the kernel does not apply (staging or LDS
 // does not fit, or more retained rows than the 3/2 padding leaves).
+// Block size of k_yepi: that of the staged strided pass, except for the
+// longest transforms, where half of it leaves LDS for the per-wave shell
+// histograms (the transforms then take two butterflies per thread).
+template <int L, int TB>
+__host__ __device__ constexpr int yepi_nt()
+{
+    return L > 1152 ? sstage_nt<L, TB>() / 2 : sstage_nt<L, TB>();
+}
+
 template <typename T, int L>
 static int yepi_grid_L(int nbins, int nout, int64_t items, int max_grid)
 {
@@ -2219,7 +2228,7 @@ static int yepi_grid_L(int nbins, int nout, int64_t items, int max_grid)
     if constexpr (TB <= 0 || (TB * sizeof(cx<T>)) % 16 != 0) {
         return 0;
     } else {
-        constexpr int NT = sstage_nt<L, TB>();
+        constexpr int NT = yepi_nt<L, TB>();
         const size_t shm = sizeof(double) * (NT / 64) * (size_t)nbins;
         if (yepi_static_lds<T, L, TB>() + shm > 160 * 1024 - 512 || nout > 2 * (L / 3) + 1) return 0;
         // a variant that spills registers (the 1024-thread tiles) is slower
@@ -2255,7 +2264,7 @@ static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, i
     if constexpr (TB <= 0 || (TB * sizeof(cx<T>)) % 16 != 0) {
         return hipErrorNotSupported;
     } else {
-        constexpr int NT = sstage_nt<L, TB>();
+        constexpr int NT = yepi_nt<L, TB>();
         StridedArgs b = a;
         b.A = A;
         if (!a.padded_ok || a.nin != L || a.njobs != 3 || a.nout > 2 * (L / 3) + 1 || grid < 8 || grid % 8) return hipErrorInvalidValue;

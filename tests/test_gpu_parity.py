@@ -75,6 +75,9 @@ CASES = {
     # a 1:1:2 box: length factors (2, 2, 1) after the lcm expansion of
     # src/MAIN.cpp:177-226, only modes with k_i divisible by L_i retained
     "noncubic_112": dict(Nx=32, Ny=32, Nz=16, Mx=48, My=48, Mz=24, Lx=2, Ly=2, Lz=1, isNotCubic=True, nu=1e-3, C_At=0.5),
+    # the y transform of the 1024^3 run (1536 points, 1025 retained k2) on
+    # a thin x-z cross-section: the fused y pass + epilogue at that length
+    "y1536_full_k2": dict(Nx=16, Ny=1024, Nz=16, nu=2e-3, C_At=0.5, force_eps=0.1),
 }
 
 
@@ -176,7 +179,8 @@ def test_velocity_step_start(kw):
 @pytest.mark.parametrize("kw", [dict(Nx=32, nu=1e-3, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2),
                                 dict(Nx=32, nu=1e-3, C_At=0.5, isSelfAdaptive=True),
                                 dict(Nx=16, Ny=16, Nz=512, nu=2e-3, C_At=0.5, omega=(0.0, 0.0, 3.0)),
-                                dict(Nx=32, Ny=32, Nz=16, Mx=48, My=48, Mz=24, Lx=2, Ly=2, Lz=1, isNotCubic=True, nu=1e-3, C_At=0.5)])
+                                dict(Nx=32, Ny=32, Nz=16, Mx=48, My=48, Mz=24, Lx=2, Ly=2, Lz=1, isNotCubic=True, nu=1e-3, C_At=0.5),
+                                dict(Nx=16, Ny=1024, Nz=16, nu=2e-3, C_At=0.5, force_eps=0.1)])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_fused_epilogue_matches_split(monkeypatch, kw, precision):
     """The DNS step's forward y pass fused with the epilogue against the two
