@@ -56,6 +56,11 @@ def lib():
         L.orc_forcing_relambda.argtypes = [P, ctypes.c_double]
         L.orc_forcing_energy_cascade.argtypes = [P, D, ctypes.c_long, ctypes.c_int]
         L.orc_init_k41.argtypes = [P, ctypes.c_int]
+        L.orc_init_random.argtypes = [P, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
+        L.orc_splitmix.argtypes = [ctypes.c_uint64]
+        L.orc_splitmix.restype = ctypes.c_uint64
+        L.orc_rand_unif.argtypes = [ctypes.c_uint64] + [ctypes.c_int] * 4
+        L.orc_rand_unif.restype = ctypes.c_double
         L.orc_init_from_physical.argtypes = [P, D, D, D, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D]
         L.orc_finish_init.argtypes = [P, ctypes.c_int]
         L.orc_nmodes.argtypes = [P]
@@ -147,6 +152,11 @@ class Oracle:
 
     def init_k41(self, nranks=4):
         self.L.orc_init_k41(self.h, nranks)
+
+    def init_random(self, seed=0, k0=4.0, urms=1.0):
+        """Seeded random-phase solenoidal field, E(k) ~ k^4 exp(-2 (k/k0)^2),
+        E_tot = 1.5 u_rms^2 (the GPU's spns_init_random restated)."""
+        self.L.orc_init_random(self.h, seed, k0, urms)
 
     def init_from_physical(self, u, v, w):
         nf = u.shape

@@ -25,9 +25,14 @@
  * reference's QA REF logs (QA/TEST_1, QA/TEST_2 per-step Time/Dt/Ek/Om/ReLambda,
  * and QA/TEST_3..9 with 4-rank K41 emulation).
  *
- * Extensions beyond the reference (documented, parity unpinned by the
- * reference's own tests): fixed time step, explicit LES Delta, constant-power
- * forcing switch, velocity-gradient skewness.
+ * Extensions beyond the reference (documented; the reference's own tests do
+ * not cover them): fixed time step, explicit LES Delta, constant-power
+ * forcing switch, velocity-gradient skewness, the seeded synthetic initial
+ * field.  Each is pinned by tests/test_oracle_*.py instead: closed forms for
+ * the fixed step and the forcing law, a numpy restatement and closed forms
+ * for the skewness sums, SplitMix64's published outputs and the field's
+ * exact per-mode amplitude, solenoidality and Hermitian plane for the
+ * initial field.
  */
 #include <math.h>
 #include <stdint.h>
@@ -944,6 +949,84 @@ void orc_init_k41(orc_t *o, int nranks)
     for (int K = 1; K <= o->last_rad; K++) K41[K] = pow(K, -5.0 / 3.0);
     orc_forcing_energy_cascade(o, K41, o->last_rad, 0);
     free(K41);
+}
+
+/* Synthetic initial field (no reference counterpart; the seeded workload
+ * input of the measured configurations).  Random-phase, solenoidal,
+ * E(k) ~ k^4 exp(-2 (k/k0)^2), scaled to E_tot = 1.5 u_rms^2.
+ *
+ * Hash: SplitMix64's output function (Steele, Lea and Flood, "Fast
+ * splittable pseudorandom number generators", OOPSLA 2014; the constants of
+ * Vigna's splitmix64.c), applied to x + 0x9E3779B97F4A7C15.  Each mode's two
+ * uniforms per component are keyed by (seed, k1, k2, k3, draw), so the field
+ * does not depend on any loop or launch order.  Gaussians by Box-Muller;
+ * the component along k is removed; each mode gets |u_hat|^2 =
+ * E(|k|) / (4 pi |k|^2) before the global scaling; the k3 = 0 plane takes
+ * the conjugate of the canonical half (k1 > 0, or k1 = 0 and k2 > 0). */
+uint64_t orc_splitmix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+double orc_rand_unif(uint64_t seed, int k1, int k2, int k3, int d)
+{
+    uint64_t key = orc_splitmix(seed) ^ orc_splitmix(((uint64_t)(uint32_t)(k1 + 65536) << 40) ^
+                                                     ((uint64_t)(uint32_t)(k2 + 65536) << 20) ^
+                                                     (uint64_t)(uint32_t)k3 ^ ((uint64_t)d << 58));
+    return ((double)(orc_splitmix(key) >> 11) + 0.5) * (1.0 / 9007199254740992.0); /* in (0,1) */
+}
+
+void orc_init_random(orc_t *o, uint64_t seed, double k0, double urms)
+{
+    const int Mx = o->p.Mx, My = o->p.My;
+    for (int c = 0; c < 3; c++) memset(o->uk[c], 0, sizeof(double) * 2 * o->nc);
+    for (int b = 0; b < My; b++)
+        for (int a = 0; a < Mx; a++)
+            for (int k3 = 0; k3 <= o->Mz2; k3++) {
+                size_t ind = CIX(o, b, a, k3);
+                if (!o->dealiased[ind]) continue;
+                int k1 = ffreq(a, Mx), k2 = ffreq(b, My);
+                int canon = k3 > 0 || k1 > 0 || (k1 == 0 && k2 > 0);
+                int mirror = (k3 == 0) && !canon && !(k1 == 0 && k2 == 0);
+                if (!canon && !mirror) continue; /* k = 0 */
+                if (mirror) { k1 = -k1; k2 = -k2; }
+                const double kk[3] = {(double)k1, (double)k2, (double)k3};
+                const double r2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+                double gr[3], gi[3];
+                for (int c = 0; c < 3; c++) {
+                    double u1 = orc_rand_unif(seed, k1, k2, k3, 2 * c), u2 = orc_rand_unif(seed, k1, k2, k3, 2 * c + 1);
+                    double rr = sqrt(-2.0 * log(u1));
+                    gr[c] = rr * cos(2 * ORC_PI * u2);
+                    gi[c] = rr * sin(2 * ORC_PI * u2);
+                }
+                double dr = (kk[0] * gr[0] + kk[1] * gr[1] + kk[2] * gr[2]) / r2;
+                double di = (kk[0] * gi[0] + kk[1] * gi[1] + kk[2] * gi[2]) / r2;
+                double nrm = 0.0;
+                for (int c = 0; c < 3; c++) {
+                    gr[c] -= dr * kk[c];
+                    gi[c] -= di * kk[c];
+                    nrm += gr[c] * gr[c] + gi[c] * gi[c];
+                }
+                if (!(nrm > 0.0)) continue;
+                double km = sqrt(r2);
+                double Ek = km * km * km * km * exp(-2.0 * (km / k0) * (km / k0));
+                double amp = sqrt(Ek / (4.0 * ORC_PI * r2) / nrm);
+                for (int c = 0; c < 3; c++) {
+                    o->uk[c][2 * ind] = gr[c] * amp;
+                    o->uk[c][2 * ind + 1] = (mirror ? -1.0 : 1.0) * gi[c] * amp;
+                }
+            }
+    double e = integrate_field(o, 0, NULL, 0, o->last_rad_max);
+    double sc = e > 0 ? sqrt(1.5 * urms * urms / e) : 0.0;
+    for (int c = 0; c < 3; c++) {
+        for (size_t q = 0; q < 2 * o->nc; q++) o->uk[c][q] *= sc;
+        memset(o->R1[c], 0, sizeof(double) * 2 * o->nc);
+    }
+    o->time = 0.0;
+    o->iter = 0;
 }
 
 /* Physical file field -> Fourier coefficients (src/HIT_LES.cpp:542-599,
