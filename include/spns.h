@@ -160,9 +160,15 @@ int spns_invariant_spectra(spns_ctx *ctx, int32_t nbins, double *spectra);
 /* Wait for the context's stream. */
 int spns_synchronize(spns_ctx *ctx);
 /* Non-blocking: *done = 1 when every step enqueued by the last spns_step call
- * has finished on the device, else 0.  No reference counterpart (the
- * serving loop polls its runs with it). */
+ * has finished on the device, else 0.  Once done, returns SPNS_ENONFINITE
+ * (with *done = 1) if the run diverged: its CFL velocity, energy, enstrophy
+ * or time step is NaN/Inf.  No reference counterpart (the serving loop polls
+ * its runs with it and retires a diverged run alone, where the reference's
+ * crash() would abort every rank, src/util.cpp:93-116). */
 int spns_query(spns_ctx *ctx, int32_t *done);
+/* Free and total device memory of a HIP device (hipMemGetInfo), for
+ * admission by HBM footprint.  No reference counterpart. */
+int spns_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes);
 /* Device time (ms) of the last spns_step call's kernels, from HIP events. */
 int spns_last_step_ms(spns_ctx *ctx, double *ms);
 /* Runs nsteps steps launched pass by pass with HIP events between the passes
@@ -174,7 +180,13 @@ int spns_profile_passes(spns_ctx *ctx, int32_t nsteps, double *ms);
 /* ---- FFT_package drop-in (src/FFTW_package.h:21-71) on device buffers ----
  * complex layout [y][x][Mz/2+1] (FFTW_MPI_TRANSPOSED, one rank),
  * real layout [x][y][2(Mz/2+1)] (src/util.h:75-80).  Both transforms are
- * unnormalised and destroy their input, like the FFTW plans they replace. */
+ * unnormalised and destroy their input, like the FFTW plans they replace.
+ * Ordering: like fftw_execute, spns_fft_r2c, spns_fft_c2r and
+ * spns_fft_normalize return with their output complete.  They run on a
+ * blocking HIP stream of their own, so work the caller queued on the legacy
+ * default stream before the call (filling the input, zeroing the output) is
+ * finished before they start; work on the caller's non-blocking streams must
+ * be synchronised by the caller first. */
 typedef struct spns_fft spns_fft;
 /* FFT_package::FFT_package / FFT_init (src/FFTW_package.cpp:47-56). */
 int spns_fft_create(int32_t Mx, int32_t My, int32_t Mz, int32_t precision, int32_t device, spns_fft **out);

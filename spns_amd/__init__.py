@@ -19,6 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspns.so")
 PREC_FP64 = 0
 PREC_FP32_SPLIT = 1
+# return codes of the C ABI (include/spns.h)
+SPNS_OK, SPNS_EINVAL, SPNS_EUNSUPPORTED, SPNS_ENOMEM, SPNS_EHIP, SPNS_ENONFINITE = range(6)
 _lib = None
 
 
@@ -56,7 +58,7 @@ EXPORTS = [
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
-    "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_last_step_ms", "spns_invariant_spectra",
+    "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_device_memory", "spns_last_step_ms", "spns_invariant_spectra",
     "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
@@ -101,6 +103,7 @@ def load_library(path=LIB_PATH):
         "spns_recalculate_velocity_antitransform": (I32, [P]),
         "spns_synchronize": (I32, [P]),
         "spns_query": (I32, [P, ctypes.POINTER(ctypes.c_int32)]),
+        "spns_device_memory": (I32, [I32, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "spns_invariant_spectra": (I32, [P, I32, D]),
         "spns_last_step_ms": (I32, [P, D]),
         "spns_profile_passes": (I32, [P, I32, D]),
@@ -161,6 +164,16 @@ def make_config(Nx, Ny=None, Nz=None, Mx=None, My=None, Mz=None, nu=1e-2, C_Smag
 def device_bytes(**kw):
     """Device memory a Solver with these arguments allocates."""
     return int(load_library().spns_device_bytes(ctypes.byref(make_config(**kw))))
+
+
+def device_memory(device=0):
+    """(free, total) bytes of a HIP device (spns_device_memory)."""
+    L = load_library()
+    fr, tot = ctypes.c_int64(), ctypes.c_int64()
+    rc = L.spns_device_memory(device, ctypes.byref(fr), ctypes.byref(tot))
+    if rc:
+        raise SpnsError(rc, L.spns_last_error(None).decode())
+    return int(fr.value), int(tot.value)
 
 
 class Solver:
@@ -247,7 +260,9 @@ class Solver:
         self._chk(self.L.spns_synchronize(self.h))
 
     def query(self):
-        """True when the steps enqueued by the last step() call have finished."""
+        """True when the steps enqueued by the last step() call have finished.
+        Raises SpnsError (code SPNS_ENONFINITE) once they have, if the run
+        diverged (NaN/Inf velocity, energy, enstrophy or time step)."""
         d = ctypes.c_int32()
         self._chk(self.L.spns_query(self.h, ctypes.byref(d)))
         return bool(d.value)

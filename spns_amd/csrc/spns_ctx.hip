@@ -399,7 +399,15 @@ struct spns_ctx {
     // inverse passes on that field, the z pass in its invariants mode (five
     // fields, forward along z), the forward x and y passes on five fields and
     // the shell sums.  out: [5][nbins].
-    void invariant_spectra(double* out)
+    // block count of the shell sums of k_inv_spectra
+    int inv_spectra_grid() const { return std::min(1024, nk1 * nk2); }
+    // device doubles the sums need: [5][nbins] results + [grid][5][nbins] partials
+    size_t inv_spectra_scratch() const { return ((size_t)inv_spectra_grid() + 1) * 5 * nbins; }
+
+    // Runs the passes and the shell sums; the [5][nbins] result lands in
+    // work[0 .. 5 nbins), the block partials after it (inv_spectra_scratch()
+    // doubles in all, fully overwritten).
+    void invariant_spectra_dev(double* work)
     {
         if (!cfg.gradient_invariants)
             throw Error(SPNS_EUNSUPPORTED, "invariant spectra need spns_config.gradient_invariants at creation");
@@ -415,11 +423,14 @@ struct spns_ctx {
         a.nk1 = nk1; a.nk2 = nk2; a.nk3 = nk3; a.nk3p = nk3p; a.nbins = nbins;
         a.notcubic = cfg.isNotCubic; a.Lx = cfg.Lx; a.Ly = cfg.Ly; a.Lz = cfg.Lz; a.Mz = Mz;
         a.inv_nf2 = 1.0 / (nf * nf);
-        const int grid = std::min(1024, nk1 * nk2);
-        // [grid][5][nbins] block partials, then the [5][nbins] sums
-        if (!invspec) alloc((void**)&invspec, sizeof(double) * ((size_t)grid + 1) * 5 * nbins);
-        a.partial = invspec + (size_t)5 * nbins;
-        HIPCHK(launch_inv_spectra(prec, a, grid, invspec, st));
+        a.partial = work + (size_t)5 * nbins;
+        HIPCHK(launch_inv_spectra(prec, a, inv_spectra_grid(), work, st));
+    }
+
+    void invariant_spectra(double* out)
+    {
+        if (!invspec) alloc((void**)&invspec, sizeof(double) * inv_spectra_scratch());
+        invariant_spectra_dev(invspec);
         HIPCHK(hipMemcpyAsync(out, invspec, sizeof(double) * 5 * nbins, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
@@ -1101,8 +1112,26 @@ int spns_query(spns_ctx* ctx, int32_t* done)
     } else {
         HIPCHK(e);
         *done = 1;
+        // the stream is idle: a small read tells whether the run diverged
+        const DevScalars s = ctx->read_scalars();
+        if (!std::isfinite(s.maxvel) || !std::isfinite(s.ek) || !std::isfinite(s.om) || !std::isfinite(s.dt))
+            throw Error(SPNS_ENONFINITE, "non-finite velocity field after step " + std::to_string(s.iter));
     }
     API_END(ctx)
+}
+
+int spns_device_memory(int32_t device, int64_t* free_bytes, int64_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return SPNS_EINVAL;
+    size_t fr = 0, tot = 0;
+    if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        (void)hipGetLastError();
+        g_create_error = "hipMemGetInfo failed on device " + std::to_string(device);
+        return SPNS_EHIP;
+    }
+    *free_bytes = (int64_t)fr;
+    *total_bytes = (int64_t)tot;
+    return SPNS_OK;
 }
 
 int spns_invariant_spectra(spns_ctx* ctx, int32_t nbins, double* spectra)
@@ -1190,7 +1219,9 @@ int spns_fft_create(int32_t Mx, int32_t My, int32_t Mz, int32_t precision, int32
         f->Mx = Mx; f->My = My; f->Mz = Mz; f->device = device;
         f->prec = precision == SPNS_PREC_FP32_SPLIT ? PREC_FP32 : PREC_FP64;
         HIPCHK(hipSetDevice(device));
-        HIPCHK(hipStreamCreateWithFlags(&f->st, hipStreamNonBlocking));
+        // a blocking stream: work the caller queued on the legacy default
+        // stream (e.g. the fill of an input) is ordered before each transform
+        HIPCHK(hipStreamCreateWithFlags(&f->st, hipStreamDefault));
         f->twx = upload_twiddles(Mx, f->prec);
         f->twy = upload_twiddles(My, f->prec);
         f->twz = upload_twiddles(Mz, f->prec);
@@ -1213,13 +1244,18 @@ int spns_fft_destroy(spns_fft* f)
     return SPNS_OK;
 }
 
+// The transforms return with their output complete, like the FFTW plans
+// they replace (src/FFTW_package.cpp:7-14 call fftw_execute, which returns
+// when done).
 int spns_fft_r2c(spns_fft* f, void* rin, void* cout)
 {
     if (!f || !rin || !cout) return SPNS_EINVAL;
     try {
+        HIPCHK(hipSetDevice(f->device));
         f->z(rin, cout, -1);
         f->fft_y(cout, -1);
         f->fft_x(cout, -1);
+        HIPCHK(hipStreamSynchronize(f->st));
     } catch (const Error& e) {
         f->err = e.what();
         return e.code;
@@ -1231,9 +1267,11 @@ int spns_fft_c2r(spns_fft* f, void* cin, void* rout)
 {
     if (!f || !cin || !rout) return SPNS_EINVAL;
     try {
+        HIPCHK(hipSetDevice(f->device));
         f->fft_x(cin, +1);
         f->fft_y(cin, +1);
         f->z(cin, rout, +1);
+        HIPCHK(hipStreamSynchronize(f->st));
     } catch (const Error& e) {
         f->err = e.what();
         return e.code;
@@ -1252,7 +1290,9 @@ int spns_fft_normalize(spns_fft* f, void* c, int32_t Nx, int32_t Ny, int32_t Nz)
         a.all = Nx <= 0;
         a.Lx = a.Ly = a.Lz = 1;
         a.nf = (double)((int64_t)f->Mx * f->My * f->Mz);
+        HIPCHK(hipSetDevice(f->device));
         HIPCHK(launch_normalize(f->prec, a, f->st));
+        HIPCHK(hipStreamSynchronize(f->st));
     } catch (const Error& e) {
         f->err = e.what();
         return e.code;

@@ -14,10 +14,15 @@ constants and rotation rates) arrive over time.  The loop
     are not held behind long ones;
   * retires runs that reached their step count (collecting E(k), energy,
     dissipation and skewness), frees their memory and admits queued runs
-    into the freed space while the others continue.
+    into the freed space while the others continue;
+  * retires a run that diverged (spns_query reports SPNS_ENONFINITE after
+    the chunk in which its field went NaN/Inf) or failed to start, with an
+    error result, and keeps the others going: one bad request never takes
+    the loop down (the reference's crash() aborts every rank instead,
+    src/util.cpp:93-116).
 Latency = completion time - arrival time; the loop reports p50/p99 beside
-throughput.  The z-pass kernel variant of each shape class is picked once by
-a small timing autotuner (Tuner).
+throughput.  The z-pass kernel variant of each shape class is picked by a
+small timing autotuner (Tuner) before the loop starts, on an idle device.
 
 Runs never share state: a run's result is the same alone and in the mix
 (tests/test_gpu_serve.py checks it bit for bit).
@@ -28,7 +33,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import PREC_FP32_SPLIT, Solver, device_bytes
+from . import PREC_FP32_SPLIT, Solver, SpnsError, device_bytes, device_memory
 
 GRIDS = [(32, 32, 32), (64, 64, 64), (96, 64, 64), (128, 128, 128), (192, 128, 128)]
 
@@ -44,15 +49,17 @@ class Request:
     steps: int
     arrival: float
     k0: float = 4.0
+    fixed_dt: float = 0.0  # > 0: a fixed time step instead of the CFL step
     # filled by the server
     admitted: float = math.nan
     finished: float = math.nan
     result: dict = field(default_factory=dict)
+    error: str = ""  # set when the run failed (diverged, or could not start)
 
     def solver_args(self, precision):
         Nx, Ny, Nz = self.grid
         return dict(Nx=Nx, Ny=Ny, Nz=Nz, nu=self.nu, C_Smag=self.C_Smag, C_At=0.5,
-                    omega=(0.0, 0.0, self.omega_z), precision=precision)
+                    omega=(0.0, 0.0, self.omega_z), precision=precision, fixed_dt=self.fixed_dt)
 
 
 def draw_requests(n, rate=20.0, seed=0, steps=(100, 2000)):
@@ -84,8 +91,19 @@ class Tuner:
         self.steps = steps
         self.choice = {}
 
+    @staticmethod
+    def key(args):
+        return (args["Nx"], args["Ny"], args["Nz"], args.get("precision", 0), args.get("C_Smag", 0.0) > 0)
+
+    def prepare(self, arg_list):
+        """Tunes every shape class of arg_list (call while the device is
+        idle, before any run is admitted: the timings then see no other
+        work, and the throwaway runs' memory is free again afterwards)."""
+        for args in arg_list:
+            self.variant(args)
+
     def variant(self, args):
-        key = (args["Nx"], args["Ny"], args["Nz"], args.get("precision", 0), args.get("C_Smag", 0.0) > 0)
+        key = self.key(args)
         if key in self.choice:
             return self.choice[key]
         best, best_ms = 0, math.inf
@@ -113,22 +131,33 @@ def step_cost(req):
     return max(pts * (1.4 if req.C_Smag > 0 else 1.0), 1e6)
 
 
-def eligible_runs(active, inflight_work, min_inflight):
-    """Indices into active (dicts with "req" and "done") of the runs that may
-    get their next chunk: all of them when inflight_work is None, else the
-    runs in ascending order of remaining work (step_cost x steps left) while
-    their per-step costs fit inflight_work, and at least min_inflight."""
+def eligible_runs(active, inflight_work, min_inflight, max_wait=None):
+    """Indices into active (dicts with "req", "done" and "waited") of the runs
+    that may get their next chunk: all of them when inflight_work is None,
+    else the runs in ascending order of remaining work (step_cost x steps
+    left) while their per-step costs fit inflight_work, and at least
+    min_inflight.  Aging: a run that has been passed over max_wait times in
+    a row ("waited", counted by the caller) is chosen as well, whatever its
+    remaining work, on top of that budget, so a long run cannot starve
+    behind a stream of short ones."""
     if inflight_work is None:
         return set(range(len(active)))
+    def overdue(i):
+        return max_wait is not None and active[i].get("waited", 0) >= max_wait
     order = sorted(range(len(active)),
-                   key=lambda i: (step_cost(active[i]["req"]) * (active[i]["req"].steps - active[i]["done"]), i))
-    chosen, budget = set(), 0.0
+                   key=lambda i: (not overdue(i),
+                                  step_cost(active[i]["req"]) * (active[i]["req"].steps - active[i]["done"]), i))
+    chosen, budget, n = set(), 0.0, 0
     for i in order:
+        if overdue(i):  # on top of the budget and of min_inflight
+            chosen.add(i)
+            continue
         c = step_cost(active[i]["req"])
-        if len(chosen) >= min_inflight and budget + c > inflight_work:
+        if n >= min_inflight and budget + c > inflight_work:
             break
         chosen.add(i)
         budget += c
+        n += 1
     return chosen
 
 
@@ -142,9 +171,9 @@ class Server:
     for a few per cent of throughput and p99."""
 
     def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=64, tuner=None,
-                 inflight_work=2e7, min_inflight=6):
+                 inflight_work=2e7, min_inflight=6, max_wait=16, device=0):
         if hbm_budget is None:
-            hbm_budget = self.free_device_memory() * 0.8
+            hbm_budget = device_memory(device)[0] * 0.8
         self.budget = hbm_budget
         self.precision = precision
         self.chunk = chunk
@@ -152,35 +181,46 @@ class Server:
         self.tuner = tuner
         self.inflight_work = inflight_work
         self.min_inflight = max(1, int(min_inflight))
-
-    @staticmethod
-    def free_device_memory():
-        try:
-            import torch
-            free, _ = torch.cuda.mem_get_info(0)
-            return float(free)
-        except Exception:
-            return 200e9
+        self.max_wait = max_wait
 
     def _start(self, req, now):
         args = req.solver_args(self.precision)
         v = self.tuner.variant(args) if self.tuner else 0
         s = Solver(zpass_variant=v, **args)
-        s.init_random(seed=req.seed, k0=req.k0)
-        s.finish_init(False)
+        try:
+            s.init_random(seed=req.seed, k0=req.k0)
+            s.finish_init(False)
+        except SpnsError:
+            s.close()
+            raise
         req.admitted = now
-        return {"req": req, "solver": s, "done": 0, "bytes": device_bytes(zpass_variant=v, **args)}
+        return {"req": req, "solver": s, "done": 0, "waited": 0, "bytes": device_bytes(zpass_variant=v, **args)}
 
     def _finish(self, a, now):
         s, req = a["solver"], a["req"]
-        sc = s.scalars()
-        req.result = {"ek": s.spectrum(), "energy": sc["ek"], "epsilon": sc["nu"] * sc["om"],
-                      "skewness": sc["skewness"], "time": sc["time"], "steps": sc["iter"], "state": s.get_state()}
+        try:
+            sc = s.scalars()
+            req.result = {"ek": s.spectrum(), "energy": sc["ek"], "epsilon": sc["nu"] * sc["om"],
+                          "skewness": sc["skewness"], "time": sc["time"], "steps": sc["iter"], "state": s.get_state()}
+        except SpnsError as e:
+            self._fail(a, e, now)
+            return
         req.finished = now
         s.close()
 
+    @staticmethod
+    def _fail(a, err, now):
+        req = a["req"]
+        req.error = str(err)
+        req.result = {"error_code": getattr(err, "code", -1), "steps": a.get("done", 0)}
+        req.finished = now
+        if a.get("solver") is not None:
+            a["solver"].close()
+
     def run(self, requests, keep_state=False):
         pending = sorted(requests, key=lambda r: r.arrival)
+        if self.tuner:
+            self.tuner.prepare([r.solver_args(self.precision) for r in pending])
         active, done = [], []
         used = 0.0
         t0 = time.perf_counter()
@@ -194,21 +234,34 @@ class Server:
                 if used + need > self.budget and active:
                     break
                 pending.pop(0)
-                a = self._start(r, now)
+                try:
+                    a = self._start(r, now)
+                except SpnsError as e:
+                    self._fail({"req": r}, e, now)
+                    done.append(r)
+                    continue
                 used += a["bytes"]
                 active.append(a)
             if not active:
-                time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                if pending:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                 continue
             # every run whose last chunk has finished gets its next chunk (or
             # retires); the others keep running: no run waits for another
-            eligible = eligible_runs(active, self.inflight_work, self.min_inflight)
+            eligible = eligible_runs(active, self.inflight_work, self.min_inflight, self.max_wait)
             progressed = False
             still = []
             for ia, a in enumerate(active):
                 s = a["solver"]
-                if a["done"] > 0 and not s.query():
-                    still.append(a)
+                try:
+                    if a["done"] > 0 and not s.query():
+                        still.append(a)
+                        continue
+                except SpnsError as e:  # diverged (or a device error) in its last chunk
+                    self._fail(a, e, time.perf_counter() - t0)
+                    used -= a["bytes"]
+                    done.append(a["req"])
+                    progressed = True
                     continue
                 if a["done"] >= a["req"].steps:
                     self._finish(a, time.perf_counter() - t0)
@@ -219,11 +272,13 @@ class Server:
                     progressed = True
                     continue
                 if ia not in eligible:
+                    a["waited"] += 1
                     still.append(a)  # waits for the shorter runs
                     continue
                 n = min(self.chunk, a["req"].steps - a["done"])
                 s.step(n)
                 a["done"] += n
+                a["waited"] = 0
                 steps_done += n
                 progressed = True
                 still.append(a)
@@ -231,11 +286,13 @@ class Server:
             if not progressed:
                 time.sleep(2e-4)
         wall = time.perf_counter() - t0
-        lat = np.array([r.finished - r.arrival for r in done])
+        ok = [r for r in done if not r.error]
+        lat = np.array([r.finished - r.arrival for r in ok])
         return {
             "requests": len(done),
+            "failed": len(done) - len(ok),
             "wall_s": wall,
-            "runs_per_s": len(done) / wall,
+            "runs_per_s": len(ok) / wall,
             "steps_per_s": steps_done / wall,
             "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else math.nan,
             "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else math.nan,

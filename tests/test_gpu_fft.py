@@ -33,6 +33,7 @@ def test_r2c_c2r_vs_numpy(shape, prec):
     rin = torch.from_numpy(rp).to(dev, dt)
     cout = torch.zeros((My, Mx, nk, 2), dtype=dt, device=dev)
     f = spns_amd.Fft(Mx, My, Mz, precision=prec)
+    torch.cuda.synchronize()  # inputs and outputs were filled on torch's stream
     f.r2c(rin.data_ptr(), cout.data_ptr())
     f.synchronize()
     c = cout.double().cpu().numpy()
@@ -42,6 +43,7 @@ def test_r2c_c2r_vs_numpy(shape, prec):
     assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < tol
     # c2r back (input destroyed, as in the reference)
     rout = torch.zeros((Mx, My, 2 * nk), dtype=dt, device=dev)
+    torch.cuda.synchronize()  # inputs and outputs were filled on torch's stream
     f.c2r(cout.data_ptr(), rout.data_ptr())
     f.synchronize()
     back = rout.double().cpu().numpy()[:, :, :Mz] / r.size
@@ -62,6 +64,7 @@ def test_c2r_single_mode_closed_form():
     ct = torch.from_numpy(np.stack([c.real, c.imag], -1)).to(dev)
     rout = torch.zeros((Mx, My, 2 * nk), dtype=torch.float64, device=dev)
     f = spns_amd.Fft(Mx, My, Mz)
+    torch.cuda.synchronize()  # inputs and outputs were filled on torch's stream
     f.c2r(ct.data_ptr(), rout.data_ptr())
     f.synchronize()
     x = np.arange(Mx)[:, None, None] * 2 * np.pi / Mx

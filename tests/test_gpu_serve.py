@@ -33,3 +33,35 @@ def test_admission_respects_budget():
         resident = [q for q in done if q.admitted <= r.admitted < q.finished]
         assert sum(fp[q.rid] for q in resident) <= budget
     assert stats["latency_p99_s"] >= stats["latency_p50_s"] > 0
+
+
+def test_diverging_run_is_isolated():
+    """One request with a fixed time step far above its CFL limit overflows
+    to Inf/NaN within a few steps.  The loop retires it with an error
+    (SPNS_ENONFINITE from spns_query) after that chunk and the other runs
+    finish, bit-identical to running alone."""
+    from spns_amd import SPNS_ENONFINITE, serve
+    reqs = serve.draw_requests(6, rate=500.0, seed=7, steps=(30, 50))
+    bad = reqs[2]
+    bad.grid, bad.C_Smag, bad.nu, bad.steps = (32, 32, 32), 0.0, 1e-4, 400
+    bad.fixed_dt = 5.0  # the CFL limit here is about 1e-2
+    srv = serve.Server(hbm_budget=40e9, chunk=10)
+    stats, done = srv.run(reqs, keep_state=True)
+    assert stats["requests"] == 6 and stats["failed"] == 1
+    by = {r.rid: r for r in done}
+    assert by[bad.rid].error and by[bad.rid].result["error_code"] == SPNS_ENONFINITE
+    assert by[bad.rid].result["steps"] < bad.steps  # stopped early, not after all its steps
+    for r in done:
+        if r.rid == bad.rid:
+            continue
+        assert not r.error
+        alone = serve.run_alone(r)
+        assert np.array_equal(alone["state"], r.result["state"]), r.rid
+        assert np.array_equal(alone["ek"], r.result["ek"])
+
+
+def test_device_memory_query():
+    from spns_amd import device_memory
+    free, total = device_memory(0)
+    assert 0 < free <= total
+    assert total > 200e9  # MI355X: 288 GB of HBM3E

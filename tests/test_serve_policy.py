@@ -40,3 +40,46 @@ def test_shortest_remaining_work_first():
 def test_finished_runs_sort_first():
     act = [run(0, (64, 64, 64), 100, done=100), run(1, (32, 32, 32), 100)]
     assert 0 in eligible_runs(act, 1.0, 1)  # no remaining work: retired on its turn
+
+
+def test_long_run_is_not_starved():
+    """A long run among a steady stream of short arrivals: with aging it gets
+    a chunk at least once every max_wait + 1 rounds; without aging it would
+    wait as long as short runs keep arriving."""
+    long = run(0, (192, 128, 128), 2000)
+    long["waited"] = 0
+    for max_wait, expect_starved in ((4, False), (None, True)):
+        act = [dict(long, done=0, waited=0)]
+        nid, gaps, last = 1, [], 0
+        for rnd in range(1, 60):
+            # two short runs arrive every round and need a few rounds each
+            for _ in range(2):
+                a = run(nid, (32, 32, 32), 40)
+                a["waited"] = 0
+                act.append(a)
+                nid += 1
+            chosen = eligible_runs(act, 4e6, 2, max_wait)
+            for i, a in enumerate(act):
+                if i in chosen:
+                    a["done"] += 20
+                    a["waited"] = 0
+                else:
+                    a["waited"] = a.get("waited", 0) + 1
+            if 0 in chosen and act[0]["req"].rid == 0:
+                gaps.append(rnd - last)
+                last = rnd
+            act = [a for a in act if a["done"] < a["req"].steps or a["req"].rid == 0]
+        if expect_starved:
+            assert not gaps  # shortest-remaining-work alone never picks the long run here
+        else:
+            assert gaps and max(gaps) <= max_wait + 1
+            assert act[0]["done"] >= 20 * (59 // (max_wait + 1))
+
+
+def test_overdue_runs_do_not_count_against_min_inflight():
+    act = [run(0, (192, 128, 128), 2000), run(1, (32, 32, 32), 100), run(2, (32, 32, 32), 100)]
+    for a in act:
+        a["waited"] = 0
+    act[0]["waited"] = 3
+    assert eligible_runs(act, 1e6, 1, max_wait=3) == {0, 1}
+    assert eligible_runs(act, 1e6, 1, max_wait=4) == {1}
