@@ -170,21 +170,27 @@ def cpu_baseline(n_cpu, threads, steps, les=False):
     return ts[len(ts) // 2], ts, lib().orc_num_threads()
 
 
-def run_ensemble(n, seed):
-    """The serving mix (Poisson arrivals at 20 req/s, mixed grids, DNS/LES,
-    viscosities and rotation rates, 100-2000 steps each, FP32-split), first n
-    requests: throughput and completion-latency percentiles."""
+def run_ensemble(n, seed, batched=True, rate=20.0):
+    """The serving mix (Poisson arrivals at `rate` req/s, mixed grids,
+    DNS/LES, viscosities and rotation rates, 100-2000 steps each,
+    FP32-split), first n requests: throughput and completion-latency
+    percentiles.  batched: one launch per pass over the runs of a shape
+    class (spns_batch); else one graph per run on its own stream."""
     from spns_amd import serve
-    reqs = serve.draw_requests(n, rate=20.0, seed=seed, steps=(100, 2000))
-    srv = serve.Server(tuner=serve.Tuner())
+    reqs = serve.draw_requests(n, rate=rate, seed=seed, steps=(100, 2000))
+    srv = serve.Server(tuner=None if batched else serve.Tuner(), batched=batched)
     stats, done = srv.run(reqs)
-    stats["policy"] = {"inflight_work": srv.inflight_work, "min_inflight": srv.min_inflight}
+    if batched:
+        stats["policy"] = {"batched": True, "nslots": srv.nslots, "chunk": srv.chunk}
+    else:
+        stats["policy"] = {"batched": False, "inflight_work": srv.inflight_work, "min_inflight": srv.min_inflight}
     steps = sum(r.steps for r in reqs)
     out = dict(stats)
-    out.update({"arrival_rate_per_s": 20.0, "seed": seed, "total_steps": steps,
+    out.update({"arrival_rate_per_s": rate, "seed": seed, "total_steps": steps,
                 "grids": sorted(set("x".join(map(str, r.grid)) for r in reqs)),
-                "les_fraction": sum(r.C_Smag > 0 for r in reqs) / max(1, len(reqs)),
-                "zpass_variant_choice": {"%s" % (k,): v for k, v in srv.tuner.choice.items()}})
+                "les_fraction": sum(r.C_Smag > 0 for r in reqs) / max(1, len(reqs))})
+    if not batched:
+        out["zpass_variant_choice"] = {"%s" % (k,): v for k, v in srv.tuner.choice.items()}
     return out
 
 

@@ -34,7 +34,8 @@ enum {
     SPNS_EUNSUPPORTED = 2, /* grid length not compiled in, or feature off */
     SPNS_ENOMEM = 3,       /* device allocation failed */
     SPNS_EHIP = 4,         /* HIP runtime error */
-    SPNS_ENONFINITE = 5    /* the CFL reduction saw a NaN/Inf velocity */
+    SPNS_ENONFINITE = 5,   /* the CFL reduction saw a NaN/Inf velocity */
+    SPNS_ECHECK = 6        /* a self-check (spns_check_phys_four_energy) failed */
 };
 
 enum { SPNS_PREC_FP64 = 0, SPNS_PREC_FP32_SPLIT = 1 };
@@ -146,6 +147,15 @@ int spns_velocity_physical(spns_ctx *ctx, double *uvw);
  * alone.  Needs a kept step-start state (keep_step_start, gradient_invariants
  * or isSelfAdaptive) once a step has run; SPNS_EUNSUPPORTED otherwise. */
 int spns_velocity_physical_step_start(spns_ctx *ctx, double *uvw);
+/* HIT::Check_PhysFour_Energy (src/HIT_LES.cpp:1216-1252), the QA build's
+ * check at the start of every step: the energy of the current state summed
+ * directly over the retained modes (ek_four), by shells (ek_spns,
+ * HIT::Recalculate_Energy) and in physical space (ek_real = 1/2 sum u.u /
+ * (Mx My Mz) over the FFT grid, FP64 sums).  Returns SPNS_ECHECK, with the
+ * reference's message, if |ek_real - ek_four| or |ek_spns - ek_four| exceeds
+ * atol (the reference crashes there).  Leaves the step's scalars, spectrum
+ * and CFL maxima unchanged.  Outputs may be NULL. */
+int spns_check_phys_four_energy(spns_ctx *ctx, double atol, double *ek_real, double *ek_four, double *ek_spns);
 /* HIT::Recalculate_Velocity_Antitransform (src/HIT_LES.cpp:60-69) without
  * copying the field out: refreshes the CFL maxima from the current state. */
 int spns_recalculate_velocity_antitransform(spns_ctx *ctx);
@@ -176,6 +186,34 @@ int spns_last_step_ms(spns_ctx *ctx, double *ms);
  * clear, inv_y, inv_x, z pass, fwd_x, fwd_y, epilogue, finalize.  The steps
  * advance the solution like spns_step (no reference counterpart). */
 int spns_profile_passes(spns_ctx *ctx, int32_t nsteps, double *ms);
+
+/* ---- Batched stepping of several runs (no reference counterpart) ----
+ * The serving loop advances independent runs of one shape class (same N, M,
+ * precision and model: DNS or LES) together: each pass of the step is one
+ * kernel launch over every slot of the batch, and a chunk of steps is one
+ * captured hipGraph.  Slots are ordinary contexts (spns_create + an initial
+ * field + spns_finish_init), attached while the batch is idle; their
+ * physics parameters (nu, C_s, rotation, forcing, time step) may differ.
+ * Each slot computes bit for bit what spns_step would on its own.  A
+ * context attached to a batch must not be stepped with spns_step, and its
+ * other calls must wait until the batch is idle (spns_batch_query reports
+ * done, or spns_batch_synchronize).  Grids whose 3/2-padded FFT lengths are
+ * 48, 96, 144, 192 or 288 have batched kernels; others return
+ * SPNS_EUNSUPPORTED at attach. */
+typedef struct spns_batch spns_batch;
+/* nslots in 1..64; chunk = the most steps one spns_batch_step takes. */
+int spns_batch_create(int32_t device, int32_t nslots, int32_t chunk, spns_batch **out);
+int spns_batch_destroy(spns_batch *b);
+int spns_batch_attach(spns_batch *b, int32_t slot, spns_ctx *ctx);
+int spns_batch_detach(spns_batch *b, int32_t slot);
+/* Advances slot s by nsteps[s] steps (0..chunk; 0 or an empty slot idles),
+ * asynchronously, on the batch's stream. */
+int spns_batch_step(spns_batch *b, const int32_t *nsteps);
+/* Non-blocking: *done = 1 when the last spns_batch_step has finished.  Then
+ * spns_query on a slot's context reports SPNS_ENONFINITE if it diverged. */
+int spns_batch_query(spns_batch *b, int32_t *done);
+int spns_batch_synchronize(spns_batch *b);
+const char *spns_batch_last_error(const spns_batch *b);
 
 /* ---- FFT_package drop-in (src/FFTW_package.h:21-71) on device buffers ----
  * complex layout [y][x][Mz/2+1] (FFTW_MPI_TRANSPOSED, one rank),

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libspns.so")
 PREC_FP64 = 0
 PREC_FP32_SPLIT = 1
 # return codes of the C ABI (include/spns.h)
-SPNS_OK, SPNS_EINVAL, SPNS_EUNSUPPORTED, SPNS_ENOMEM, SPNS_EHIP, SPNS_ENONFINITE = range(6)
+SPNS_OK, SPNS_EINVAL, SPNS_EUNSUPPORTED, SPNS_ENOMEM, SPNS_EHIP, SPNS_ENONFINITE, SPNS_ECHECK = range(7)
 _lib = None
 
 
@@ -58,10 +58,13 @@ EXPORTS = [
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
-    "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_device_memory", "spns_last_step_ms", "spns_invariant_spectra",
+    "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_check_phys_four_energy",
+    "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_device_memory", "spns_last_step_ms", "spns_invariant_spectra",
     "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
+    "spns_batch_create", "spns_batch_destroy", "spns_batch_attach", "spns_batch_detach", "spns_batch_step",
+    "spns_batch_query", "spns_batch_synchronize", "spns_batch_last_error",
 ]
 
 
@@ -101,6 +104,7 @@ def load_library(path=LIB_PATH):
         "spns_velocity_physical": (I32, [P, D]),
         "spns_velocity_physical_step_start": (I32, [P, D]),
         "spns_recalculate_velocity_antitransform": (I32, [P]),
+        "spns_check_phys_four_energy": (I32, [P, ctypes.c_double, D, D, D]),
         "spns_synchronize": (I32, [P]),
         "spns_query": (I32, [P, ctypes.POINTER(ctypes.c_int32)]),
         "spns_device_memory": (I32, [I32, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
@@ -114,6 +118,14 @@ def load_library(path=LIB_PATH):
         "spns_fft_normalize": (I32, [P, P, I32, I32, I32]),
         "spns_fft_synchronize": (I32, [P]),
         "spns_fft_last_error": (ctypes.c_char_p, [P]),
+        "spns_batch_create": (I32, [I32, I32, I32, ctypes.POINTER(P)]),
+        "spns_batch_destroy": (I32, [P]),
+        "spns_batch_attach": (I32, [P, I32, P]),
+        "spns_batch_detach": (I32, [P, I32]),
+        "spns_batch_step": (I32, [P, ctypes.POINTER(I32)]),
+        "spns_batch_query": (I32, [P, ctypes.POINTER(I32)]),
+        "spns_batch_synchronize": (I32, [P]),
+        "spns_batch_last_error": (ctypes.c_char_p, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -283,6 +295,14 @@ class Solver:
     def recalc_timestep(self):
         self._chk(self.L.spns_recalculate_timestep(self.h))
 
+    def check_energy(self, atol=1e-13):
+        """HIT::Check_PhysFour_Energy: (EkReal, EkFour, EkSpNS) of the current
+        state; raises SpnsError (SPNS_ECHECK) if they differ by more than atol."""
+        er, ef, es = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        rc = self.L.spns_check_phys_four_energy(self.h, atol, ctypes.byref(er), ctypes.byref(ef), ctypes.byref(es))
+        self._chk(rc)
+        return er.value, ef.value, es.value
+
     def antitransform(self):
         """Refresh the CFL maxima from the current state (no copy-out)."""
         self._chk(self.L.spns_recalculate_velocity_antitransform(self.h))
@@ -376,6 +396,61 @@ class Fft:
     def close(self):
         if getattr(self, "h", None):
             self.L.spns_fft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched stepping of up to nslots runs of one shape class
+    (spns_batch_*): each pass of a step is one kernel launch over all
+    attached runs; a chunk of up to `chunk` steps is one captured graph."""
+
+    def __init__(self, nslots, chunk, device=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.spns_batch_create(device, nslots, chunk, ctypes.byref(h))
+        if rc:
+            raise SpnsError(rc, self.L.spns_batch_last_error(None).decode())
+        self.h = h
+        self.nslots, self.chunk = nslots, chunk
+        self.solvers = [None] * nslots
+
+    def _chk(self, rc):
+        if rc:
+            raise SpnsError(rc, self.L.spns_batch_last_error(self.h).decode())
+
+    def attach(self, slot, solver):
+        self._chk(self.L.spns_batch_attach(self.h, slot, solver.h))
+        self.solvers[slot] = solver
+
+    def detach(self, slot):
+        self._chk(self.L.spns_batch_detach(self.h, slot))
+        self.solvers[slot] = None
+
+    def free_slots(self):
+        return [i for i, s in enumerate(self.solvers) if s is None]
+
+    def step(self, nsteps):
+        """nsteps: per-slot step counts (0..chunk)."""
+        n = (ctypes.c_int32 * self.nslots)(*[int(v) for v in nsteps])
+        self._chk(self.L.spns_batch_step(self.h, n))
+
+    def query(self):
+        d = ctypes.c_int32()
+        self._chk(self.L.spns_batch_query(self.h, ctypes.byref(d)))
+        return bool(d.value)
+
+    def synchronize(self):
+        self._chk(self.L.spns_batch_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.spns_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -111,6 +111,7 @@ HIT::HIT(const ptrdiff_t& Nx_, const ptrdiff_t& Ny_, const ptrdiff_t& Nz_, const
     // rest of HIT_init (src/HIT_LES.cpp:1037-1086)
     check(spns_finish_init(ctx, isReLambda_), "HIT_init");
     Ek_init = Recalculate_Energy();
+    qa_check = opt.qa_check;
 }
 
 HIT::~HIT()
@@ -189,8 +190,14 @@ void HIT::read_input_file(std::vector<double>& u, std::vector<double>& v, std::v
             }
 }
 
+void HIT::Check_PhysFour_Energy(const double atol)
+{
+    check(spns_check_phys_four_energy(ctx, atol, nullptr, nullptr, nullptr), "Check_PhysFour_Energy");
+}
+
 void HIT::New_Fractional_Step_Method()
 {
+    if (qa_check) Check_PhysFour_Energy(1e-13);  // src/HIT_LES.cpp:10-12 (QA build)
     check(spns_step(ctx, 1), "New_Fractional_Step_Method");
     phys_current = false;  // the step's u,v,w are those of its start state
 }

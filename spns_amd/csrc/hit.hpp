@@ -59,6 +59,7 @@ struct Options {
     bool use_graph = true;
     bool gradient_invariants = false;  // enables Recalculate_Invariants_Distribution
     bool keep_step_start = false;      // RealVelocity_to_BinaryFile between steps (the step-start field)
+    bool qa_check = false;             // the QA build's Check_PhysFour_Energy(1e-13) before every step
 };
 
 // HIT (src/HIT_LES.h:21-246).
@@ -87,6 +88,7 @@ public:
     double Recalculate_Energy();                            // :1254-1259
     double Recalculate_Energy_Cascade(const char* filename = nullptr);  // :1261-1266
     void Recalculate_Velocity_Antitransform();              // :60-69
+    void Check_PhysFour_Energy(const double atol);          // :1216-1252 (QA)
     void Recalculate_Invariants_Distribution(const char* filename[5] = nullptr);  // :1297-1319
     void FourierVelocity_to_BinaryFile(char const* const filename) const;  // src/HIT_LES_output.cpp:8-10
     void RealVelocity_to_BinaryFile(char const* const filename) const;     // src/HIT_LES_output.cpp:12-14
@@ -125,6 +127,7 @@ private:
     double nu, ReLambda = 0.0, Ok_Tot = 0.0, Ek_Tot = 0.0, Ek_init = 0.0, Ek_init_file = 0.0, Enstrophy_init_file = 0.0;
     mutable double At = 0.0, time = 0.0;
     bool phys_current = true;  // u,v,w of the current state (else of the last step's start)
+    bool qa_check = false;     // Options::qa_check
     std::vector<double> Ek, VelGradInv;
 };
 

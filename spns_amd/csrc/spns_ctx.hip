@@ -169,8 +169,10 @@ struct spns_ctx {
     DevScalars* scal = nullptr;
     double* hist_partial = nullptr;
     double* om_partial = nullptr;
+    double* ed_partial = nullptr;  // [grid_epi] direct E sums of a reduce (QA energy check)
     double* skew_partial = nullptr;
     unsigned long long* mv = nullptr;  // [2]
+    unsigned long long* mv_save = nullptr;  // [2] (the QA energy check keeps the CFL maxima)
     double* ek = nullptr;
     double* forced = nullptr;
     double* ft = nullptr;
@@ -270,8 +272,10 @@ struct spns_ctx {
         }
         alloc((void**)&hist_partial, sizeof(double) * (size_t)grid_epi * nbins);
         alloc((void**)&om_partial, sizeof(double) * grid_epi);
+        alloc((void**)&ed_partial, sizeof(double) * grid_epi);
         alloc((void**)&skew_partial, sizeof(double) * 2 * grid_z);
         alloc((void**)&mv, sizeof(unsigned long long) * 2);
+        alloc((void**)&mv_save, sizeof(unsigned long long) * 2);
         alloc((void**)&ek, sizeof(double) * nbins);
         alloc((void**)&ft, sizeof(double) * (nbins + 1));
         HIPCHK(hipMemsetAsync(skew_partial, 0, sizeof(double) * 2 * grid_z, st));
@@ -316,7 +320,7 @@ struct spns_ctx {
         if (graph) hipGraphExecDestroy(graph);
         for (int c = 0; c < 3; c++) { hipFree(uk[c]); hipFree(R[c]); hipFree(ukprev[c]); }
         hipFree(X1); hipFree(X2); hipFree(twx); hipFree(twy); hipFree(twz);
-        hipFree(scal); hipFree(hist_partial); hipFree(om_partial); hipFree(skew_partial); hipFree(mv);
+        hipFree(scal); hipFree(hist_partial); hipFree(om_partial); hipFree(ed_partial); hipFree(skew_partial); hipFree(mv); hipFree(mv_save);
         hipFree(ek); hipFree(forced); hipFree(ft); hipFree(invspec);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
@@ -325,7 +329,10 @@ struct spns_ctx {
 
     // ---------------------------------------------------------------- passes
     // y passes: rows = y (or i2), batch = (i1 outer, i3 contiguous)
-    void inv_y(bool velocity_only, cx<double>* const* src = nullptr)
+    // Each pass's argument struct (with the outer extent A set) is built by
+    // its *_args method, so the batched step (spns_batch) launches exactly
+    // the kernels the single-run step does, on the same structs.
+    StridedArgs inv_y_args(bool velocity_only, cx<double>* const* src = nullptr)
     {
         if (!src) src = uk;
         StridedArgs a{};
@@ -340,11 +347,17 @@ struct spns_ctx {
         int64_t icap[9], ocap[9];
         for (int j = 0; j < n; j++) { icap[j] = nmodes_p; ocap[j] = x1_slot; }
         check_strided(a, nk1, icap, ocap, strided_tile(My, prec));
-        HIPCHK(launch_strided(My, prec, 1, prec == PREC_FP64, +1, a, nk1, st));
+        a.A = nk1;
+        return a;
+    }
+    void inv_y(bool velocity_only, cx<double>* const* src = nullptr)
+    {
+        const StridedArgs a = inv_y_args(velocity_only, src);
+        HIPCHK(launch_strided(My, prec, 1, prec == PREC_FP64, +1, a, a.A, st));
     }
 
     // x passes: rows = x (or i1), batch = (y outer, i3 contiguous)
-    void inv_x(bool velocity_only)
+    StridedArgs inv_x_args(bool velocity_only)
     {
         StridedArgs a{};
         int n = 0;
@@ -360,10 +373,16 @@ struct spns_ctx {
         int64_t icap[9], ocap[9];
         for (int j = 0; j < n; j++) { icap[j] = x1_slot; ocap[j] = x2_slot; }
         check_strided(a, My, icap, ocap, strided_tile(Mx, prec));
-        HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, +1, a, My, st));
+        a.A = My;
+        return a;
+    }
+    void inv_x(bool velocity_only)
+    {
+        const StridedArgs a = inv_x_args(velocity_only);
+        HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, +1, a, a.A, st));
     }
 
-    void zpass(int mode, void* phys)
+    ZArgs zpass_args(void* phys)
     {
         ZArgs z{};
         for (int f = 0; f < 9; f++) z.f[f] = X2f(f);
@@ -381,6 +400,11 @@ struct spns_ctx {
         z.phys = phys;
         z.tw = twz;
         if (z.npencil * nk3p > x2_slot || nk3 > nk3p) throw Error(SPNS_EINVAL, "z pass exceeds its buffer");
+        return z;
+    }
+    void zpass(int mode, void* phys)
+    {
+        ZArgs z = zpass_args(phys);
         if (mode == 2) {
             z.kernel = ZK_STAGED_NOFUSE;
             const hipError_t e = launch_zpass(Mz, prec, mode, z, grid_z, st);
@@ -435,7 +459,7 @@ struct spns_ctx {
         HIPCHK(hipStreamSynchronize(st));
     }
 
-    void fwd_x(int nfields = 0)
+    StridedArgs fwd_x_args(int nfields = 0)
     {
         const int nf = nfields > 0 ? nfields : nfwd;
         StridedArgs a{};
@@ -448,7 +472,13 @@ struct spns_ctx {
         int64_t icap[9], ocap[9];
         for (int j = 0; j < nf; j++) { icap[j] = x2_slot; ocap[j] = x1_slot; }
         check_strided(a, My, icap, ocap, strided_tile(Mx, prec));
-        HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, My, st));
+        a.A = My;
+        return a;
+    }
+    void fwd_x(int nfields = 0)
+    {
+        const StridedArgs a = fwd_x_args(nfields);
+        HIPCHK(launch_strided(Mx, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, a.A, st));
     }
 
     StridedArgs fwd_y_args(int nf)
@@ -464,13 +494,14 @@ struct spns_ctx {
         for (int j = 0; j < nf; j++) { icap[j] = x1_slot; ocap[j] = nmodes_p; }
         check_strided(a, nk1, icap, ocap, strided_tile(My, prec));
         if ((int64_t)nf * nmodes_p > 9 * x2_slot) throw Error(SPNS_EINVAL, "compact fields exceed X2");
+        a.A = nk1;
         return a;
     }
 
     void fwd_y(int nfields = 0)
     {
         const StridedArgs a = fwd_y_args(nfields > 0 ? nfields : nfwd);
-        HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, nk1, st));
+        HIPCHK(launch_strided(My, prec, prec == PREC_FP64, prec == PREC_FP64, -1, a, a.A, st));
     }
 
     // the DNS step's forward y pass fused with its epilogue (k_yepi) when
@@ -503,6 +534,7 @@ struct spns_ctx {
         e.scal = scal;
         e.hist_partial = hist_partial;
         e.om_partial = om_partial;
+        e.ed_partial = mode == EPI_REDUCE ? ed_partial : nullptr;
         return e;
     }
 
@@ -512,7 +544,7 @@ struct spns_ctx {
         f.scal = scal;
         f.flags = flags;
         f.nbins = nbins; f.nblk_epi = grid_epi; f.nblk_z = grid_z;
-        f.hist_partial = hist_partial; f.om_partial = om_partial; f.skew_partial = skew_partial;
+        f.hist_partial = hist_partial; f.om_partial = om_partial; f.ed_partial = ed_partial; f.skew_partial = skew_partial;
         f.maxvel_bits = mv; f.maxvel_sa_bits = mv + 1;
         f.ek = ek;
         f.kfmin = cfg.force_kmin; f.kfmax = cfg.force_kmax; f.force_eps = cfg.force_on ? cfg.force_eps : 0.0;
@@ -544,7 +576,30 @@ struct spns_ctx {
     void reduce_state()
     {
         HIPCHK(launch_epilogue(prec, epi_args(EPI_REDUCE), grid_epi, st));
-        HIPCHK(launch_finalize(fin_args(FIN_REDUCE), st));
+        HIPCHK(launch_finalize(fin_args(FIN_REDUCE | FIN_EFOUR), st));
+    }
+
+    // HIT::Check_PhysFour_Energy (src/HIT_LES.cpp:1216-1252) on the current
+    // state: E summed directly over the retained modes (EkFour), by shells
+    // (EkSpNS) and in physical space (EkReal, 1/2 sum u.u / (Mx My Mz) from
+    // a velocity pass).  The step's scalars, spectrum and CFL maxima are
+    // left as they were.
+    void check_phys_four_energy(double* ek_real, double* ek_four, double* ek_spns)
+    {
+        const DevScalars before = read_scalars();
+        std::vector<double> bins(nbins);
+        HIPCHK(hipMemcpyAsync(bins.data(), ek, sizeof(double) * nbins, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(mv_save, mv, sizeof(unsigned long long) * 2, hipMemcpyDeviceToDevice, st));
+        reduce_state();
+        velocity_pass(nullptr);
+        const DevScalars after = read_scalars();
+        *ek_real = 0.5 * after.ek_phys / nf;  // src/HIT_LES.cpp:1236-1240
+        *ek_four = after.ek_four;
+        *ek_spns = after.ek;
+        write_scalars(before);
+        HIPCHK(hipMemcpyAsync(ek, bins.data(), sizeof(double) * nbins, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(mv, mv_save, sizeof(unsigned long long) * 2, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
     }
 
     // u,v,w from the state: CFL maxima (and the physical field if phys)
@@ -569,7 +624,7 @@ struct spns_ctx {
         inv_y(true);
         inv_x(true);
         zpass(1, phys);
-        HIPCHK(launch_finalize(fin_args(FIN_MAXVEL), st));
+        HIPCHK(launch_finalize(fin_args(FIN_MAXVEL | FIN_EPHYS), st));
     }
 
     void step(int n)
@@ -694,6 +749,199 @@ struct spns_ctx {
         s.nk1 = nk1; s.nk2 = nk2; s.nk3 = nk3; s.nk3p = nk3p; s.nmodes = nmodes_p;
         s.notcubic = cfg.isNotCubic; s.Lx = cfg.Lx; s.Ly = cfg.Ly; s.Lz = cfg.Lz;
         return s;
+    }
+};
+
+// ===========================================================================
+// Batched stepping of several runs of one shape class (spns_batch_*).
+// Each slot is an ordinary context (its own buffers, created and initialised
+// as usual).  A batched step launches every pass once over all slots: the
+// kernels take the slots' argument structs from device arrays (slot =
+// blockIdx.y), built by the same *_args methods the single-run step uses,
+// with the single run's grids, so every slot computes exactly what it would
+// alone.  Per-slot steps-left counters let slots advance different step
+// counts within one captured chunk; a slot at 0 skips every launch.
+// ===========================================================================
+struct spns_batch {
+    int device = 0, nslots = 0, chunk = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<spns_ctx*> slot;
+    spns_ctx* rep = nullptr;  // a slot context giving the class's shapes
+    // per-slot argument structs, host copies and device arrays
+    std::vector<StridedArgs> h_invy, h_invx, h_fwdx, h_fwdy;
+    std::vector<ZArgs> h_z;
+    std::vector<EpiArgs> h_epi;
+    std::vector<FinArgs> h_fin;
+    std::vector<ClearArgs> h_clr;
+    StridedArgs *d_invy = nullptr, *d_invx = nullptr, *d_fwdx = nullptr, *d_fwdy = nullptr;
+    ZArgs* d_z = nullptr;
+    EpiArgs* d_epi = nullptr;
+    FinArgs* d_fin = nullptr;
+    ClearArgs* d_clr = nullptr;
+    int* d_left = nullptr;
+    hipGraphExec_t graph = nullptr;
+    std::string err;
+
+    void init(int dev, int n, int ch)
+    {
+        if (n < 1 || n > kMaxBatchSlots) throw Error(SPNS_EINVAL, "nslots must be in 1.." + std::to_string(kMaxBatchSlots));
+        if (ch < 1) throw Error(SPNS_EINVAL, "chunk must be >= 1");
+        device = dev; nslots = n; chunk = ch;
+        slot.assign(n, nullptr);
+        h_invy.assign(n, StridedArgs{}); h_invx.assign(n, StridedArgs{}); h_fwdx.assign(n, StridedArgs{});
+        h_fwdy.assign(n, StridedArgs{}); h_z.assign(n, ZArgs{}); h_epi.assign(n, EpiArgs{});
+        h_fin.assign(n, FinArgs{}); h_clr.assign(n, ClearArgs{});
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        auto al = [&](void** p, size_t bytes) {
+            hipError_t e = hipMalloc(p, bytes);
+            if (e != hipSuccess) { *p = nullptr; throw Error(SPNS_ENOMEM, "batch argument arrays: " + std::string(hipGetErrorString(e))); }
+        };
+        al((void**)&d_invy, sizeof(StridedArgs) * n); al((void**)&d_invx, sizeof(StridedArgs) * n);
+        al((void**)&d_fwdx, sizeof(StridedArgs) * n); al((void**)&d_fwdy, sizeof(StridedArgs) * n);
+        al((void**)&d_z, sizeof(ZArgs) * n); al((void**)&d_epi, sizeof(EpiArgs) * n);
+        al((void**)&d_fin, sizeof(FinArgs) * n); al((void**)&d_clr, sizeof(ClearArgs) * n);
+        al((void**)&d_left, sizeof(int) * n);
+        SlotSteps z{};
+        HIPCHK(launch_set_left(d_left, z, nslots, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void release()
+    {
+        if (st) hipStreamSynchronize(st);
+        if (graph) hipGraphExecDestroy(graph);
+        hipFree(d_invy); hipFree(d_invx); hipFree(d_fwdx); hipFree(d_fwdy); hipFree(d_z); hipFree(d_epi);
+        hipFree(d_fin); hipFree(d_clr); hipFree(d_left);
+        if (ev) hipEventDestroy(ev);
+        if (st) hipStreamDestroy(st);
+    }
+
+    // a context joins the class if its shapes, model and kernel choices match
+    // the representative's
+    static bool same_class(const spns_ctx* a, const spns_ctx* b)
+    {
+        return a->Nx == b->Nx && a->Ny == b->Ny && a->Nz == b->Nz && a->Mx == b->Mx && a->My == b->My && a->Mz == b->Mz &&
+               a->prec == b->prec && a->les == b->les && a->nfwd == b->nfwd && a->zkernel == b->zkernel &&
+               a->grid_z == b->grid_z && a->grid_epi == b->grid_epi && a->grid_yepi == b->grid_yepi && a->nbins == b->nbins &&
+               a->cfg.device == b->cfg.device;
+    }
+
+    void attach(int s, spns_ctx* c)
+    {
+        if (s < 0 || s >= nslots) throw Error(SPNS_EINVAL, "slot out of range");
+        if (!c) throw Error(SPNS_EINVAL, "null context");
+        if (c->cfg.device != device) throw Error(SPNS_EINVAL, "context on another device");
+        spns_ctx* r = rep;
+        if (!r)
+            for (spns_ctx* o : slot) if (o) { r = o; break; }
+        if (r && !same_class(r, c)) throw Error(SPNS_EINVAL, "context does not match the batch's shape class");
+        if (!batch_length(c->Mx) || !batch_length(c->My) || !batch_length(c->Mz))
+            throw Error(SPNS_EUNSUPPORTED, "no batched kernels for this grid's FFT lengths");
+        HIPCHK(hipStreamSynchronize(st));        // no chunk in flight reads the arrays
+        HIPCHK(hipStreamSynchronize(c->st));     // the context's own work is done
+        h_invy[s] = c->inv_y_args(false);
+        h_invx[s] = c->inv_x_args(false);
+        h_z[s] = c->zpass_args(nullptr);
+        h_fwdx[s] = c->fwd_x_args();
+        h_fwdy[s] = c->fwd_y_args(c->nfwd);
+        h_epi[s] = c->epi_args(EPI_STEP);
+        FinArgs f = c->fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE);
+        f.nblk_epi = c->grid_yepi > 0 ? c->grid_yepi : c->grid_epi;
+        h_fin[s] = f;
+        h_clr[s] = ClearArgs{c->mv, c->mv + 1};
+        auto up = [&](void* d, const void* h, size_t bytes) { HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice)); };
+        up(d_invy + s, &h_invy[s], sizeof(StridedArgs)); up(d_invx + s, &h_invx[s], sizeof(StridedArgs));
+        up(d_z + s, &h_z[s], sizeof(ZArgs)); up(d_fwdx + s, &h_fwdx[s], sizeof(StridedArgs));
+        up(d_fwdy + s, &h_fwdy[s], sizeof(StridedArgs)); up(d_epi + s, &h_epi[s], sizeof(EpiArgs));
+        up(d_fin + s, &h_fin[s], sizeof(FinArgs)); up(d_clr + s, &h_clr[s], sizeof(ClearArgs));
+        slot[s] = c;
+        rep = r ? r : c;
+    }
+
+    void detach(int s)
+    {
+        if (s < 0 || s >= nslots) throw Error(SPNS_EINVAL, "slot out of range");
+        HIPCHK(hipStreamSynchronize(st));
+        slot[s] = nullptr;
+        bool any = false;
+        for (spns_ctx* o : slot) any = any || o;
+        if (!any) rep = nullptr;
+        else if (rep == nullptr || std::find(slot.begin(), slot.end(), rep) == slot.end())
+            for (spns_ctx* o : slot) if (o) { rep = o; break; }
+    }
+
+    // one step of every live slot: the single-run step's launches
+    // (spns_ctx::enqueue_step), each once over all slots
+    void enqueue_step()
+    {
+        spns_ctx* c = rep;
+        const int p = c->prec;
+        auto bl = [&](const void* a, const void* a2 = nullptr) { return BatchLaunch{a, a2, d_left, nslots}; };
+        BatchLaunch b;
+        b = bl(d_clr);
+        HIPCHK(launch_clear(nullptr, nullptr, st, &b));
+        b = bl(d_invy);
+        HIPCHK(launch_strided(c->My, p, 1, p == PREC_FP64, +1, h_invy[slot_of(c)], c->nk1, st, &b));
+        b = bl(d_invx);
+        HIPCHK(launch_strided(c->Mx, p, p == PREC_FP64, p == PREC_FP64, +1, h_invx[slot_of(c)], c->My, st, &b));
+        b = bl(d_z);
+        HIPCHK(launch_zpass(c->Mz, p, 0, h_z[slot_of(c)], c->grid_z, st, &b));
+        b = bl(d_fwdx);
+        HIPCHK(launch_strided(c->Mx, p, p == PREC_FP64, p == PREC_FP64, -1, h_fwdx[slot_of(c)], c->My, st, &b));
+        if (c->grid_yepi > 0) {
+            b = bl(d_fwdy, d_epi);
+            HIPCHK(launch_yepi(c->My, p, h_fwdy[slot_of(c)], c->nk1, h_epi[slot_of(c)], c->grid_yepi, st, &b));
+        } else {
+            b = bl(d_fwdy);
+            HIPCHK(launch_strided(c->My, p, p == PREC_FP64, p == PREC_FP64, -1, h_fwdy[slot_of(c)], c->nk1, st, &b));
+            b = bl(d_epi);
+            HIPCHK(launch_epilogue(p, h_epi[slot_of(c)], c->grid_epi, st, &b));
+        }
+        b = bl(d_fin);
+        HIPCHK(launch_finalize(h_fin[slot_of(c)], st, &b));
+    }
+
+    int slot_of(const spns_ctx* c) const
+    {
+        for (int s = 0; s < nslots; s++)
+            if (slot[s] == c) return s;
+        return 0;
+    }
+
+    void step(const int32_t* nsteps)
+    {
+        if (!rep) throw Error(SPNS_EINVAL, "no context attached");
+        SlotSteps n{};
+        for (int s = 0; s < nslots; s++) {
+            const int v = slot[s] ? nsteps[s] : 0;
+            if (v < 0 || v > chunk) throw Error(SPNS_EINVAL, "steps per slot must be in 0..chunk");
+            n.n[s] = v;
+        }
+        HIPCHK(hipSetDevice(device));
+        if (!graph) {
+            // the chunk's launches read only the device arrays and counters,
+            // so one capture serves every later set of slots
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int i = 0; i < chunk; i++) enqueue_step();
+            } catch (...) {
+                hipGraph_t dummy;
+                hipStreamEndCapture(st, &dummy);
+                if (dummy) hipGraphDestroy(dummy);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(st, &g));
+            const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            HIPCHK(e);
+        }
+        HIPCHK(launch_set_left(d_left, n, nslots, st));
+        HIPCHK(hipGraphLaunch(graph, st));
+        HIPCHK(hipEventRecord(ev, st));
     }
 };
 
@@ -1087,6 +1335,23 @@ int spns_velocity_physical_step_start(spns_ctx* ctx, double* uvw)
     API_END(ctx)
 }
 
+int spns_check_phys_four_energy(spns_ctx* ctx, double atol, double* ek_real, double* ek_four, double* ek_spns)
+{
+    API_BEGIN(ctx)
+    double er = 0.0, ef = 0.0, es = 0.0;
+    ctx->check_phys_four_energy(&er, &ef, &es);
+    if (ek_real) *ek_real = er;
+    if (ek_four) *ek_four = ef;
+    if (ek_spns) *ek_spns = es;
+    const double d1 = std::fabs(er - ef), d2 = std::fabs(es - ef);
+    if (!(d1 <= atol) || !(d2 <= atol)) {  // src/HIT_LES.cpp:1246-1250
+        char msg[256];
+        std::snprintf(msg, sizeof msg, "Ek differences exceed tolerance [EkReal vs EkFour %e, EkSpNS vs EkFour: %e]", d1, d2);
+        throw Error(SPNS_ECHECK, msg);
+    }
+    API_END(ctx)
+}
+
 int spns_recalculate_velocity_antitransform(spns_ctx* ctx)
 {
     API_BEGIN(ctx)
@@ -1312,5 +1577,89 @@ int spns_fft_synchronize(spns_fft* f)
 }
 
 const char* spns_fft_last_error(const spns_fft* f) { return f ? f->err.c_str() : g_create_error.c_str(); }
+
+// ------------------------------ batched stepping ------------------------------
+#define BATCH_BEGIN(b) \
+    if (!(b)) return SPNS_EINVAL; \
+    try {
+#define BATCH_END(b)                                                   \
+    }                                                                  \
+    catch (const Error& e) { (b)->err = e.what(); return e.code; }     \
+    catch (const std::exception& e) { (b)->err = e.what(); return SPNS_EINVAL; } \
+    return SPNS_OK;
+
+int spns_batch_create(int32_t device, int32_t nslots, int32_t chunk, spns_batch** out)
+{
+    if (!out) return SPNS_EINVAL;
+    *out = nullptr;
+    spns_batch* b = new spns_batch();
+    try {
+        b->init(device, nslots, chunk);
+    } catch (const Error& e) {
+        g_create_error = e.what();
+        b->release();
+        delete b;
+        return e.code;
+    }
+    *out = b;
+    return SPNS_OK;
+}
+
+int spns_batch_destroy(spns_batch* b)
+{
+    if (!b) return SPNS_EINVAL;
+    hipSetDevice(b->device);
+    b->release();
+    delete b;
+    return SPNS_OK;
+}
+
+int spns_batch_attach(spns_batch* b, int32_t slot, spns_ctx* ctx)
+{
+    BATCH_BEGIN(b)
+    HIPCHK(hipSetDevice(b->device));
+    b->attach(slot, ctx);
+    BATCH_END(b)
+}
+
+int spns_batch_detach(spns_batch* b, int32_t slot)
+{
+    BATCH_BEGIN(b)
+    HIPCHK(hipSetDevice(b->device));
+    b->detach(slot);
+    BATCH_END(b)
+}
+
+int spns_batch_step(spns_batch* b, const int32_t* nsteps)
+{
+    BATCH_BEGIN(b)
+    if (!nsteps) throw Error(SPNS_EINVAL, "nsteps is null");
+    b->step(nsteps);
+    BATCH_END(b)
+}
+
+int spns_batch_query(spns_batch* b, int32_t* done)
+{
+    BATCH_BEGIN(b)
+    if (!done) throw Error(SPNS_EINVAL, "done is null");
+    const hipError_t e = hipEventQuery(b->ev);
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();
+        *done = 0;
+    } else {
+        HIPCHK(e);
+        *done = 1;
+    }
+    BATCH_END(b)
+}
+
+int spns_batch_synchronize(spns_batch* b)
+{
+    BATCH_BEGIN(b)
+    HIPCHK(hipStreamSynchronize(b->st));
+    BATCH_END(b)
+}
+
+const char* spns_batch_last_error(const spns_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
 
 }  // extern "C"

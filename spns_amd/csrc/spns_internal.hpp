@@ -19,7 +19,9 @@ struct DevScalars {
     double ek, om, relambda;
     double skew2, skew3;
     int64_t iter;
-    double pad[3];
+    double ek_phys;  // sum of u.u over the FFT grid of the last velocity pass (FIN_EPHYS)
+    double ek_four;  // E summed directly over the retained modes (FIN_EFOUR)
+    double pad[1];
 };
 
 struct StridedJob {
@@ -55,7 +57,7 @@ struct ZArgs {
     double ct;          // (C_Smag*delta)^2 (src/HIT_LES.cpp:132)
     unsigned long long* maxvel_bits;
     unsigned long long* maxvel_sa_bits;
-    double* skew_partial;  // [gridDim.x][2]
+    double* skew_partial;  // [gridDim.x][2]: MODE 0 sums of (du/dx)^2, (du/dx)^3; MODE 1 sum of u.u, 0
     void* phys;            // MODE 1: optional [3][npencil][L] physical export
     const void* tw;
 };
@@ -84,6 +86,7 @@ struct EpiArgs {
     const DevScalars* scal;
     double* hist_partial;  // [grid][nbins]
     double* om_partial;    // [grid]
+    double* ed_partial;    // [grid] E summed directly, EPI_REDUCE only (nullptr: not kept)
 };
 
 struct InvSpecArgs {
@@ -94,7 +97,7 @@ struct InvSpecArgs {
     double* partial;   // [grid][5][nbins]
 };
 
-enum { FIN_REDUCE = 1, FIN_SKEW = 2, FIN_MAXVEL = 4, FIN_DT = 8, FIN_ADVANCE = 16 };
+enum { FIN_REDUCE = 1, FIN_SKEW = 2, FIN_MAXVEL = 4, FIN_DT = 8, FIN_ADVANCE = 16, FIN_EFOUR = 32, FIN_EPHYS = 64 };
 
 struct FinArgs {
     DevScalars* scal;
@@ -102,7 +105,8 @@ struct FinArgs {
     int nbins, nblk_epi, nblk_z;
     const double* hist_partial;
     const double* om_partial;
-    const double* skew_partial;
+    const double* ed_partial;    // FIN_EFOUR: [nblk_epi] direct E partials
+    const double* skew_partial;  // [nblk_z][2]; after a velocity pass [b][0] holds sum u.u (FIN_EPHYS)
     const unsigned long long* maxvel_bits;
     const unsigned long long* maxvel_sa_bits;
     double* ek;  // [nbins]
@@ -138,23 +142,55 @@ struct RandArgs {
     double k0;
 };
 
+struct ClearArgs {
+    unsigned long long* a;
+    unsigned long long* b;
+};
+
+// A batched launch: one kernel over the slots of a shape class (runs with
+// the same grid, precision and model).  Slot = blockIdx.y; its argument
+// struct is args[slot] (args2[slot]: k_yepi's epilogue struct) in device
+// memory; a slot whose steps-left counter left[slot] is 0 skips the launch,
+// and the batched finalize decrements it.  The grid in x is the single run's,
+// so a slot's arithmetic (and its reductions' order) is that of the run
+// stepped alone.
+struct BatchLaunch {
+    const void* args;
+    const void* args2;
+    int* left;
+    int nslots;
+};
+
+constexpr int kMaxBatchSlots = 64;
+struct SlotSteps {
+    int32_t n[kMaxBatchSlots];  // steps each slot takes in the next batched chunk
+};
+
+// FFT lengths with batched kernel instances (the serving grids' 3/2-padded
+// lengths)
+constexpr bool batch_length(int L) { return L == 48 || L == 96 || L == 144 || L == 192 || L == 288; }
+
 bool supported_length(int L);
 hipError_t launch_init_random(const RandArgs& a, hipStream_t st);
 hipError_t launch_scale_state(const ShellArgs& a, double sc, hipStream_t st);
 int strided_tile(int L, int prec);
-hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st);
-hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);
-hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st);
-hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st);
+hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st, const BatchLaunch* bl = nullptr);
+hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st,
+                               const BatchLaunch* bl = nullptr);
+hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st,
+                          const BatchLaunch* bl = nullptr);
+hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st, const BatchLaunch* bl = nullptr);
 int zpass_grid(int L, int prec, const ZArgs& a, int64_t npen);  // < 0: no such length / HIP error
 int zpass_kernel(int L, int prec, const ZArgs& a);              // kernel launch_zpass picks
 hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid, hipStream_t st);
 int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid);
-hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st);
-hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st);
-hipError_t launch_finalize(const FinArgs& a, hipStream_t st);
+hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st,
+                       const BatchLaunch* bl = nullptr);
+hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st, const BatchLaunch* bl = nullptr);
+hipError_t launch_finalize(const FinArgs& a, hipStream_t st, const BatchLaunch* bl = nullptr);
 hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* out, hipStream_t st);
-hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st);
+hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st, const BatchLaunch* bl = nullptr);
+hipError_t launch_set_left(int* left, const SlotSteps& s, int nslots, hipStream_t st);
 hipError_t launch_forcing(const double* ek, const double* forced, double* ft, int last_eff, const ShellArgs& a, hipStream_t st);
 hipError_t launch_ccc(const ShellArgs& a, hipStream_t st);
 hipError_t launch_normalize(int prec, const NormArgs& a, hipStream_t st);

@@ -147,7 +147,7 @@ __host__ __device__ constexpr bool sstage_fits()
 // of each stage-0 input (stored low rows, stored high rows, zero padding) is
 // then fixed at compile time.
 template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
-__global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
+__device__ __forceinline__ void sstage_body(const StridedArgs& a)
 {
     constexpr int SROWS = sstage_rows<L, DIR>();
     constexpr int SPR = TB * (int)sizeof(cx<TIN>) / 16;  // 16-byte copy slots per staged row
@@ -274,6 +274,19 @@ __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
         write_tile(dual ? a.job[jb].out2 : a.job[jb].out);
         lds_barrier();  // every wave's copies are in; fbuf is free again
     }
+}
+
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
+__global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
+{
+    sstage_body<TIN, TOUT, T, L, TB, NT, DIR, STDP>(a);
+}
+
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
+__global__ __launch_bounds__(NT) void k_sstage_b(const StridedArgs* __restrict__ arr, const int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    sstage_body<TIN, TOUT, T, L, TB, NT, DIR, STDP>(arr[blockIdx.y]);
 }
 
 // ---------------------------------------------------------------------------
@@ -418,6 +431,7 @@ __global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
             mw = fmax(mw, fabs(w));
             if (sa) vmax_sa = fmax(vmax_sa, fabs((double)u) / iAx + fabs((double)v) / iAy + fabs((double)w) / iAz);  // src/HIT_LES.cpp:377
             if constexpr (MODE == 1) {
+                s2 += (double)u * (double)u + (double)v * (double)v + (double)w * (double)w;  // src/HIT_LES.cpp:1236-1239
                 if (a.phys) {
                     // export physical velocity as [c][y][x][z]
                     const int64_t P = a.npencil * L;
@@ -492,14 +506,14 @@ __global__ __launch_bounds__(NT) void k_zpass(ZArgs a)
     const double bw = block_max<NT>((double)mw, red);
     const double vmax = fmax(bu / iAx, fmax(bv / iAy, bw / iAz));  // src/HIT_LES.cpp:393
     if (sa) vmax_sa = block_max<NT>(vmax_sa, red);
-    if (MODE == 0) {
+    if (MODE <= 1) {
         s2 = block_sum<NT>(s2, red);
         s3 = block_sum<NT>(s3, red);
     }
     if (t == 0) {
         atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
         atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
-        if (MODE == 0) {
+        if (MODE <= 1) {
             a.skew_partial[2 * blockIdx.x] = s2;
             a.skew_partial[2 * blockIdx.x + 1] = s3;
         }
@@ -631,7 +645,7 @@ __host__ __device__ constexpr bool zstage_fits()
 // stage resolves at compile time and its zero inputs fold away.
 template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int MINW = 1, bool ZOPAQUE = (sizeof(T) == 4),
           int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2)>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void k_zstage(ZArgs a)
+__device__ __forceinline__ void zstage_body(const ZArgs& a)
 {
     static_assert(LD >= (MODE == 1 ? 2 : 6), "pair buffer pitch too small");
     constexpr int NSRC = (MODE == 1) ? 3 : 9;
@@ -736,6 +750,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
             mw = fmax(mw, fabs(w));
             if (sa) vmax_sa = fmax(vmax_sa, fabs((double)u) / iAx + fabs((double)v) / iAy + fabs((double)w) / iAz);  // src/HIT_LES.cpp:377
             if constexpr (MODE == 1) {
+                s2 += (double)u * (double)u + (double)v * (double)v + (double)w * (double)w;  // src/HIT_LES.cpp:1236-1239
                 if (a.phys) {
                     const int64_t P = a.npencil * L;
                     T* o = (T*)a.phys;
@@ -816,18 +831,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
     const double bw = block_max<NT>((double)mw, red);
     const double vmax = fmax(bu / iAx, fmax(bv / iAy, bw / iAz));  // src/HIT_LES.cpp:393
     if (sa) vmax_sa = block_max<NT>(vmax_sa, red);
-    if (MODE == 0) {
+    if (MODE <= 1) {
         s2 = block_sum<NT>(s2, red);
         s3 = block_sum<NT>(s3, red);
     }
     if (t == 0) {
         atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
         atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
-        if (MODE == 0) {
+        if (MODE <= 1) {
             a.skew_partial[2 * blockIdx.x] = s2;
             a.skew_partial[2 * blockIdx.x + 1] = s3;
         }
     }
+}
+
+template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int MINW = 1, bool ZOPAQUE = (sizeof(T) == 4),
+          int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2)>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void k_zstage(ZArgs a)
+{
+    zstage_body<T, L, NT, MODE, LD, MINW, ZOPAQUE, NKC, TWM>(a);
+}
+
+template <typename T, int L, int NT, int MODE, int LD = zstage_ld<MODE>(), int MINW = 1, bool ZOPAQUE = (sizeof(T) == 4),
+          int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2)>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void k_zstage_b(const ZArgs* __restrict__ arr,
+                                                                                           const int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    zstage_body<T, L, NT, MODE, LD, MINW, ZOPAQUE, NKC, TWM>(arr[blockIdx.y]);
 }
 
 // ---------------------------------------------------------------------------
@@ -858,7 +889,7 @@ __host__ __device__ constexpr size_t zfuse_lds_bytes()
 }
 
 template <typename T, int L, int NT, int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2), bool ZOPAQUE = (sizeof(T) == 4)>
-__global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
+__device__ __forceinline__ void zfuse_body(const ZArgs& a)
 {
     constexpr int LD = 9, NB = 8;
     constexpr int SPC = zstage_spc<L>();
@@ -1069,6 +1100,19 @@ __global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
     }
 }
 
+template <typename T, int L, int NT, int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2), bool ZOPAQUE = (sizeof(T) == 4)>
+__global__ __launch_bounds__(NT) void k_zfuse(ZArgs a)
+{
+    zfuse_body<T, L, NT, NKC, TWM, ZOPAQUE>(a);
+}
+
+template <typename T, int L, int NT, int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2), bool ZOPAQUE = (sizeof(T) == 4)>
+__global__ __launch_bounds__(NT) void k_zfuse_b(const ZArgs* __restrict__ arr, const int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    zfuse_body<T, L, NT, NKC, TWM, ZOPAQUE>(arr[blockIdx.y]);
+}
+
 // ---------------------------------------------------------------------------
 // z pass, one wave per pencil (DNS and velocity-only modes, L = 64*E).
 // Lane t owns the physical points z = t + 64 m; the twelve fields arrive pair
@@ -1165,6 +1209,7 @@ __global__ __launch_bounds__(256) void k_zwave(ZArgs a)
                 mu = fmax(mu, fabs(U[m]));
                 mv = fmax(mv, fabs(V[m]));
                 mw = fmax(mw, fabs(W[m]));
+                s2 += (double)U[m] * (double)U[m] + (double)V[m] * (double)V[m] + (double)W[m] * (double)W[m];  // src/HIT_LES.cpp:1236-1239
             }
             if (sa) {
                 for (int m = 0; m < E; m++)
@@ -1222,14 +1267,12 @@ __global__ __launch_bounds__(256) void k_zwave(ZArgs a)
     const double bw = block_max<256>((double)mw, red);
     const double vmax = fmax(bu / iAx, fmax(bv / iAy, bw / iAz));  // src/HIT_LES.cpp:393
     if (sa) vmax_sa = block_max<256>(vmax_sa, red);
-    if (MODE == 0) {
-        s2 = block_sum<256>(s2, red);
-        s3 = block_sum<256>(s3, red);
-    }
+    s2 = block_sum<256>(s2, red);
+    s3 = block_sum<256>(s3, red);
     if (threadIdx.x == 0) {
         atomicMax(a.maxvel_bits, (unsigned long long)__double_as_longlong(vmax));
         atomicMax(a.maxvel_sa_bits, (unsigned long long)__double_as_longlong(vmax_sa));
-        if (MODE == 0) {
+        {
             a.skew_partial[2 * blockIdx.x] = s2;
             a.skew_partial[2 * blockIdx.x + 1] = s3;
         }
@@ -1334,7 +1377,8 @@ static __device__ __forceinline__ EpiScalars epi_scalars(const EpiArgs& a)
 }
 
 static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScalars& q, int64_t m, int k1, int k2, int k3,
-                                         bool step, const cx<double>* C, const cx<double>* S, double* myh, double& om)
+                                         bool step, const cx<double>* C, const cx<double>* S, double* myh, double& om,
+                                         double& ed)
 {
     const int ik2 = k1 * k1 + k2 * k2 + k3 * k3;
     const double rad2 = (double)ik2;
@@ -1421,6 +1465,7 @@ static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScala
     atomicAdd(&myh[K], conj ? 2 * ke : ke);
     const double en = 2 * rad2 * ke;
     om += conj ? 2 * en : en;
+    ed += conj ? 2 * ke : ke;
 }
 
 // Per-wave shell histograms [NW][nbins] summed in wave order into the
@@ -1441,7 +1486,7 @@ __device__ __forceinline__ void epi_flush(const EpiArgs& a, const double* hist, 
 }
 
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
+__device__ __forceinline__ void epilogue_body(const EpiArgs& a)
 {
     constexpr int NW = NT / 64;
     extern __shared__ double hist[];  // NW * nbins
@@ -1451,7 +1496,7 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
     __syncthreads();
     double* myh = hist + (threadIdx.x / 64) * nb;
     const EpiScalars q = epi_scalars(a);
-    double om = 0.0;
+    double om = 0.0, ed = 0.0;
     const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3, nk3p = a.nk3p;
     const double inv_nf = 1.0 / a.nf;
     const bool step = a.mode == EPI_STEP;
@@ -1480,9 +1525,27 @@ __global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
                 }
             }
         }
-        epi_mode(a, q, m, k1, k2, k3, step, C, (step && a.les) ? S : nullptr, myh, om);
+        epi_mode(a, q, m, k1, k2, k3, step, C, (step && a.les) ? S : nullptr, myh, om, ed);
     }
     epi_flush<NT>(a, hist, om, red);
+    if (a.ed_partial) {
+        // the direct sum of E (HIT::Check_PhysFour_Energy's EkFour, src/HIT_LES.cpp:1218-1230)
+        ed = block_sum<NT>(ed, red);
+        if (threadIdx.x == 0) a.ed_partial[blockIdx.x] = ed;
+    }
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_epilogue(EpiArgs a)
+{
+    epilogue_body<T, NT>(a);
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_epilogue_b(const EpiArgs* __restrict__ arr, const int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    epilogue_body<T, NT>(arr[blockIdx.y]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1502,7 +1565,7 @@ __host__ __device__ constexpr size_t yepi_static_lds()
 }
 
 template <typename T, int L, int TB, int NT>
-__global__ __launch_bounds__(NT) void k_yepi(StridedArgs a, EpiArgs e)
+__device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e)
 {
     constexpr int SPR = TB * (int)sizeof(cx<T>) / 16;  // 16-byte copy slots per staged row
     constexpr int EPS = 16 / (int)sizeof(cx<T>);
@@ -1523,7 +1586,7 @@ __global__ __launch_bounds__(NT) void k_yepi(StridedArgs a, EpiArgs e)
     __syncthreads();
     double* myh = hist + (t >> 6) * nb;
     const EpiScalars q = epi_scalars(e);
-    double om = 0.0;
+    double om = 0.0, ed_unused = 0.0;
     const double inv_nf = 1.0 / e.nf;
     const int nout = a.nout, hout = nout / 2;
     const int ntiles = (a.B + TB - 1) / TB;
@@ -1583,7 +1646,7 @@ __global__ __launch_bounds__(NT) void k_yepi(StridedArgs a, EpiArgs e)
                                 cx<double>{(double)keep[1][qq].x * inv_nf, (double)keep[1][qq].y * inv_nf},
                                 cx<double>{(double)v2.x * inv_nf, (double)v2.y * inv_nf}};  // Fourier_Normalization
                             const int64_t m = ((int64_t)i * e.nk1 + i1) * e.nk3p + k3;
-                            epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om);
+                            epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused);
                         }
                     }
                 }
@@ -1593,6 +1656,20 @@ __global__ __launch_bounds__(NT) void k_yepi(StridedArgs a, EpiArgs e)
         static_for<3>(comp);
     }
     epi_flush<NT>(e, hist, om, (double*)fbuf);
+}
+
+template <typename T, int L, int TB, int NT>
+__global__ __launch_bounds__(NT) void k_yepi(StridedArgs a, EpiArgs e)
+{
+    yepi_body<T, L, TB, NT>(a, e);
+}
+
+template <typename T, int L, int TB, int NT>
+__global__ __launch_bounds__(NT) void k_yepi_b(const StridedArgs* __restrict__ arr, const EpiArgs* __restrict__ earr,
+                                               const int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    yepi_body<T, L, TB, NT>(arr[blockIdx.y], earr[blockIdx.y]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1609,7 +1686,7 @@ static __device__ double FuG(double x, double a, double b, double c, double x0, 
 // Finalize: fixed-order sums of the partials; time step (src/HIT_LES.cpp:365-403)
 // ---------------------------------------------------------------------------
 template <int NT>
-__global__ __launch_bounds__(NT) void k_finalize(FinArgs a)
+__device__ __forceinline__ void finalize_body(const FinArgs& a)
 {
     __shared__ double red[NT];
     DevScalars* s = a.scal;
@@ -1667,6 +1744,18 @@ __global__ __launch_bounds__(NT) void k_finalize(FinArgs a)
         double S3 = block_sum<NT>((s3[0] + s3[1]) + (s3[2] + s3[3]), red);
         if (t == 0) { s->skew2 = S2; s->skew3 = S3; }
     }
+    if (a.flags & FIN_EFOUR) {
+        double e = 0.0;
+        for (int b = t; b < a.nblk_epi; b += NT) e += a.ed_partial[b];
+        e = block_sum<NT>(e, red);
+        if (t == 0) s->ek_four = e;
+    }
+    if (a.flags & FIN_EPHYS) {
+        double e = 0.0;
+        for (int b = t; b < a.nblk_z; b += NT) e += a.skew_partial[2 * b];
+        e = block_sum<NT>(e, red);
+        if (t == 0) s->ek_phys = e;
+    }
     if (t == 0) {
         if (a.flags & FIN_MAXVEL) {
             s->maxvel = __longlong_as_double((long long)*a.maxvel_bits);
@@ -1704,9 +1793,39 @@ __global__ __launch_bounds__(NT) void k_finalize(FinArgs a)
     }
 }
 
+template <int NT>
+__global__ __launch_bounds__(NT) void k_finalize(FinArgs a)
+{
+    finalize_body<NT>(a);
+}
+
+// Batched form: the slot's step ends here, so its steps-left counter drops
+// by one (after every wave of the block has read it at entry).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_finalize_b(const FinArgs* __restrict__ arr, int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    finalize_body<NT>(arr[blockIdx.y]);
+    __syncthreads();
+    if (threadIdx.x == 0) left[blockIdx.y] -= 1;
+}
+
 static __global__ void k_clear(unsigned long long* a, unsigned long long* b)
 {
     if (threadIdx.x == 0) { *a = 0ull; *b = 0ull; }
+}
+
+static __global__ void k_clear_b(const ClearArgs* __restrict__ arr, const int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    if (threadIdx.x == 0) { *arr[blockIdx.y].a = 0ull; *arr[blockIdx.y].b = 0ull; }
+}
+
+// Steps-left counters of a batch for its next chunk, from the kernel's
+// arguments (so the host may reuse its copy right after the launch).
+static __global__ void k_set_left(int* left, SlotSteps s, int nslots)
+{
+    if ((int)threadIdx.x < nslots) left[threadIdx.x] = s.n[threadIdx.x];
 }
 
 // Spectrum imposition (src/HIT_LES.cpp:410-444): FT[K] from E(K), then scale.
@@ -1797,7 +1916,7 @@ constexpr int zpass_nt()
 }
 
 template <typename TIN, typename TOUT, typename T, int L, int DIR>
-static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
+static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st, const BatchLaunch* bl)
 {
     constexpr int TB = strided_tb<T, L>();
     StridedArgs b = a;
@@ -1821,6 +1940,22 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
             int64_t g = (int64_t)ncu * occ;
             if (items < g) g = items;
             g = (g + 7) / 8 * 8;
+            if (bl) {
+                // batched: each slot's struct (with A set) is in bl->args
+                if constexpr (batch_length(L)) {
+                    const dim3 grid((unsigned)g, (unsigned)bl->nslots);
+                    const StridedArgs* arr = (const StridedArgs*)bl->args;
+                    if constexpr (STDP_OK) {
+                        if (stdp) {
+                            hipLaunchKernelGGL((k_sstage_b<TIN, TOUT, T, L, TB, NT, DIR, true>), grid, dim3(NT), 0, st, arr, bl->left);
+                            return hipGetLastError();
+                        }
+                    }
+                    hipLaunchKernelGGL((k_sstage_b<TIN, TOUT, T, L, TB, NT, DIR>), grid, dim3(NT), 0, st, arr, bl->left);
+                    return hipGetLastError();
+                }
+                return hipErrorNotSupported;
+            }
             if constexpr (STDP_OK) {
                 if (stdp) {
                     hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR, true>), dim3((unsigned)g), dim3(NT), 0, st, b);
@@ -1831,6 +1966,7 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st)
             return hipGetLastError();
         }
     }
+    if (bl) return hipErrorNotSupported;  // no batched form of the direct-load kernel
     // the direct-load kernel takes single jobs: a dual job becomes two
     int n = 0;
     for (int j = 0; j < a.njobs; j++) {
@@ -1876,41 +2012,42 @@ bool supported_length(int L)
 #endif
 
 template <typename TIN, typename TOUT, typename T, int DIR>
-static hipError_t launch_strided_T(int L, const StridedArgs& a, int A, hipStream_t st)
+static hipError_t launch_strided_T(int L, const StridedArgs& a, int A, hipStream_t st, const BatchLaunch* bl)
 {
 #define X(LL) \
-    if (L == LL) return launch_strided_L<TIN, TOUT, T, LL, DIR>(a, A, st);
+    if (L == LL) return launch_strided_L<TIN, TOUT, T, LL, DIR>(a, A, st, bl);
     SPNS_FOR_EACH_L(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
 #if SPNS_PART == 0 || SPNS_PART == 1
-hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st)
+hipError_t launch_strided_fp64(int L, int dir, const StridedArgs& a, int A, hipStream_t st, const BatchLaunch* bl)
 {
-    return dir > 0 ? launch_strided_T<double, double, double, +1>(L, a, A, st)
-                   : launch_strided_T<double, double, double, -1>(L, a, A, st);
+    return dir > 0 ? launch_strided_T<double, double, double, +1>(L, a, A, st, bl)
+                   : launch_strided_T<double, double, double, -1>(L, a, A, st, bl);
 }
 #endif
 
 #if SPNS_PART == 0 || SPNS_PART == 2
-hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st)
+hipError_t launch_strided_fp32(int L, int in_fp64, int dir, const StridedArgs& a, int A, hipStream_t st, const BatchLaunch* bl)
 {
     if (in_fp64) {
-        return dir > 0 ? launch_strided_T<double, float, float, +1>(L, a, A, st)
-                       : launch_strided_T<double, float, float, -1>(L, a, A, st);
+        return dir > 0 ? launch_strided_T<double, float, float, +1>(L, a, A, st, bl)
+                       : launch_strided_T<double, float, float, -1>(L, a, A, st, bl);
     }
-    return dir > 0 ? launch_strided_T<float, float, float, +1>(L, a, A, st)
-                   : launch_strided_T<float, float, float, -1>(L, a, A, st);
+    return dir > 0 ? launch_strided_T<float, float, float, +1>(L, a, A, st, bl)
+                   : launch_strided_T<float, float, float, -1>(L, a, A, st, bl);
 }
 #endif
 
 #if SPNS_PART == 0 || SPNS_PART == 4
-hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st)
+hipError_t launch_strided(int L, int prec, int in_fp64, int out_fp64, int dir, const StridedArgs& a, int A, hipStream_t st,
+                          const BatchLaunch* bl)
 {
     (void)out_fp64;
-    if (prec == PREC_FP64) return launch_strided_fp64(L, dir, a, A, st);
-    return launch_strided_fp32(L, in_fp64, dir, a, A, st);
+    if (prec == PREC_FP64) return launch_strided_fp64(L, dir, a, A, st, bl);
+    return launch_strided_fp32(L, in_fp64, dir, a, A, st, bl);
 }
 #endif
 
@@ -1974,10 +2111,42 @@ static int zpass_pick(const ZArgs& a)
 }
 
 template <typename T, int L, int MODE>
-static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st)
+static hipError_t launch_zpass_L(const ZArgs& a, int grid, hipStream_t st, const BatchLaunch* bl)
 {
     constexpr int NTS = zstage_nt<T, L>();
     constexpr int NKC = zstage_nkc<L>();
+    if (bl) {
+        // batched step (MODE 0) over the slots' structs in bl->args
+        if constexpr (MODE == 0 && batch_length(L)) {
+            const dim3 g((unsigned)grid, (unsigned)bl->nslots);
+            const ZArgs* arr = (const ZArgs*)bl->args;
+            switch (zpass_pick<T, L, MODE>(a)) {
+            case ZV_FUSE_NKC:
+                if constexpr (zfuse_fits<T, L>()) {
+                    constexpr int NTF = zfuse_nt<T, L>();
+                    hipLaunchKernelGGL((k_zfuse_b<T, L, NTF, NKC>), g, dim3(NTF), 0, st, arr, bl->left);
+                    return hipGetLastError();
+                }
+                break;
+            case ZV_STAGE_NKC:
+                if constexpr (NKC > 0 && zstage_fits<T, L, NTS, MODE>()) {
+                    hipLaunchKernelGGL((k_zstage_b<T, L, NTS, MODE, zstage_ld<MODE>(), 1, sizeof(T) == 4, NKC>), g, dim3(NTS), 0, st,
+                                       arr, bl->left);
+                    return hipGetLastError();
+                }
+                break;
+            case ZV_STAGE:
+                if constexpr (zstage_fits<T, L, NTS, MODE>()) {
+                    hipLaunchKernelGGL((k_zstage_b<T, L, NTS, MODE>), g, dim3(NTS), 0, st, arr, bl->left);
+                    return hipGetLastError();
+                }
+                break;
+            default:
+                break;
+            }
+        }
+        return hipErrorNotSupported;
+    }
     switch (zpass_pick<T, L, MODE>(a)) {
     case ZV_WAVE:
         if constexpr (has_wave_plan<L>::value) {
@@ -2086,17 +2255,17 @@ int zpass_kernel(int L, int prec, const ZArgs& a)
     return -1;
 }
 
-hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st)
+hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hipStream_t st, const BatchLaunch* bl)
 {
 #define X(LL)                                                                                  \
     if (L == LL) {                                                                             \
         if (prec == PREC_FP64)                                                                 \
-            return mode == 0   ? launch_zpass_L<double, LL, 0>(a, grid, st)                    \
-                   : mode == 1 ? launch_zpass_L<double, LL, 1>(a, grid, st)                    \
-                               : launch_zpass_L<double, LL, 2>(a, grid, st);                   \
-        return mode == 0   ? launch_zpass_L<float, LL, 0>(a, grid, st)                         \
-               : mode == 1 ? launch_zpass_L<float, LL, 1>(a, grid, st)                         \
-                           : launch_zpass_L<float, LL, 2>(a, grid, st);                        \
+            return mode == 0   ? launch_zpass_L<double, LL, 0>(a, grid, st, bl)                \
+                   : mode == 1 ? launch_zpass_L<double, LL, 1>(a, grid, st, bl)                \
+                               : launch_zpass_L<double, LL, 2>(a, grid, st, bl);               \
+        return mode == 0   ? launch_zpass_L<float, LL, 0>(a, grid, st, bl)                     \
+               : mode == 1 ? launch_zpass_L<float, LL, 1>(a, grid, st, bl)                     \
+                           : launch_zpass_L<float, LL, 2>(a, grid, st, bl);                    \
     }
     SPNS_FOR_EACH_L(X)
 #undef X
@@ -2258,7 +2427,7 @@ int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid)
 }
 
 template <typename T, int L>
-static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st)
+static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st, const BatchLaunch* bl)
 {
     constexpr int TB = strided_tb<T, L>();
     if constexpr (TB <= 0 || (TB * sizeof(cx<T>)) % 16 != 0) {
@@ -2269,38 +2438,67 @@ static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, i
         b.A = A;
         if (!a.padded_ok || a.nin != L || a.njobs != 3 || a.nout > 2 * (L / 3) + 1 || grid < 8 || grid % 8) return hipErrorInvalidValue;
         const size_t shm = sizeof(double) * (NT / 64) * (size_t)e.nbins;
+        if (bl) {
+            if constexpr (batch_length(L)) {
+                hipLaunchKernelGGL((k_yepi_b<T, L, TB, NT>), dim3(grid, bl->nslots), dim3(NT), shm, st, (const StridedArgs*)bl->args,
+                                   (const EpiArgs*)bl->args2, bl->left);
+                return hipGetLastError();
+            }
+            return hipErrorNotSupported;
+        }
         hipLaunchKernelGGL((k_yepi<T, L, TB, NT>), dim3(grid), dim3(NT), shm, st, b, e);
         return hipGetLastError();
     }
 }
 
-hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st)
+hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st, const BatchLaunch* bl)
 {
 #define X(LL) \
-    if (L == LL) return prec == PREC_FP64 ? launch_yepi_L<double, LL>(a, A, e, grid, st) : launch_yepi_L<float, LL>(a, A, e, grid, st);
+    if (L == LL) return prec == PREC_FP64 ? launch_yepi_L<double, LL>(a, A, e, grid, st, bl) : launch_yepi_L<float, LL>(a, A, e, grid, st, bl);
     SPNS_FOR_EACH_L(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st)
+hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st, const BatchLaunch* bl)
 {
     constexpr int NT = 256;
     const size_t shm = sizeof(double) * (NT / 64) * a.nbins;
+    if (bl) {
+        const EpiArgs* arr = (const EpiArgs*)bl->args;
+        if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue_b<double, NT>), dim3(grid, bl->nslots), dim3(NT), shm, st, arr, bl->left);
+        else hipLaunchKernelGGL((k_epilogue_b<float, NT>), dim3(grid, bl->nslots), dim3(NT), shm, st, arr, bl->left);
+        return hipGetLastError();
+    }
     if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue<double, NT>), dim3(grid), dim3(NT), shm, st, a);
     else hipLaunchKernelGGL((k_epilogue<float, NT>), dim3(grid), dim3(NT), shm, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const FinArgs& a, hipStream_t st)
+hipError_t launch_finalize(const FinArgs& a, hipStream_t st, const BatchLaunch* bl)
 {
+    if (bl) {
+        hipLaunchKernelGGL((k_finalize_b<1024>), dim3(1, bl->nslots), dim3(1024), 0, st, (const FinArgs*)bl->args, bl->left);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((k_finalize<1024>), dim3(1), dim3(1024), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st)
+hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st, const BatchLaunch* bl)
 {
+    if (bl) {
+        hipLaunchKernelGGL(k_clear_b, dim3(1, bl->nslots), dim3(64), 0, st, (const ClearArgs*)bl->args, bl->left);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_left(int* left, const SlotSteps& s, int nslots, hipStream_t st)
+{
+    if (nslots < 1 || nslots > kMaxBatchSlots) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_set_left, dim3(1), dim3(kMaxBatchSlots), 0, st, left, s, nslots);
     return hipGetLastError();
 }
 

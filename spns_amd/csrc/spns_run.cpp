@@ -8,7 +8,10 @@
 // reference's order (src/MAIN.cpp:413-485): step, first-iteration Re_lambda
 // forcing, spectrum forcing, periodic Re_lambda forcing, iteration count,
 // logs, outputs.  With --qa (or QA 1 in the file) every step prints the QA
-// log row of src/MAIN.cpp:440.  Output files use the reference's names.
+// log row of src/MAIN.cpp:440 and, as the reference's QA build does, first
+// checks that the physical, spectral and shell-sum energies agree to 1e-13
+// (HIT::Check_PhysFour_Energy); a violation ends the run with status 1.
+// Output files use the reference's names.
 //
 // Optional keys beyond the reference: Precision (fp64 | fp32split),
 // Fixed_Timestep, Max_Iterations, Delta, Constant_Power (injection rate),
@@ -133,6 +136,7 @@ int main(int argc, char** argv)
         opt.k41_ranks = (int)PM.integer("K41_Ranks", 1);
         opt.gradient_invariants = VelGradInv == 1;
         opt.keep_step_start = Vel_Phys == 1 && Vel_Phys_Freq > 0;  // in-loop dumps write the step-start field
+        opt.qa_check = qa;  // the QA build checks the energy before every step (src/HIT_LES.cpp:10-12)
         const long max_iter = PM.integer("Max_Iterations", 0);
         static const char* const inv_names[5] = {"QG", "QS", "RG", "RS", "V2"};
         char inv_fname[5][1024];

@@ -33,7 +33,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import PREC_FP32_SPLIT, Solver, SpnsError, device_bytes, device_memory
+from . import PREC_FP32_SPLIT, Batch, Solver, SpnsError, device_bytes, device_memory
 
 GRIDS = [(32, 32, 32), (64, 64, 64), (96, 64, 64), (128, 128, 128), (192, 128, 128)]
 
@@ -161,17 +161,35 @@ def eligible_runs(active, inflight_work, min_inflight, max_wait=None):
     return chosen
 
 
-class Server:
-    """inflight_work: None advances every admitted run as soon as its
-    previous chunk finishes (processor sharing).  A number (the default,
-    2e7) gives the next chunks to the runs with the least remaining work
-    (step_cost x steps left) first, as many as fit that per-step cost budget
-    (at least min_inflight), so short runs finish early and long ones use the
-    slack.  On the bench mix this cuts the median latency about threefold
-    for a few per cent of throughput and p99."""
+def fft_points(req):
+    Nx, Ny, Nz = req.grid
+    return (3 * Nx // 2) * (3 * Ny // 2) * (3 * Nz // 2)
 
-    def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=64, tuner=None,
-                 inflight_work=2e7, min_inflight=6, max_wait=16, device=0):
+
+def class_key(req, precision):
+    """Shape class of a request: runs of one class share batched launches."""
+    return (tuple(req.grid), precision, req.C_Smag > 0)
+
+
+class Server:
+    """batched=True (the default): runs of one shape class (grid, precision,
+    DNS or LES) share a Batch of nslots slots, and each pass of a step is one
+    kernel launch over every run of the batch; a chunk of up to `chunk`
+    steps is one captured graph, and the batches of different classes run
+    concurrently on their own streams.  Within a batch every run advances
+    min(chunk, steps left) per chunk; a class whose batches are full opens
+    another batch.  Runs wait (admitted, initialised) for a slot until their
+    batch is between chunks.
+
+    batched=False: one graph per run on its own stream.  inflight_work: None
+    advances every admitted run as soon as its previous chunk finishes
+    (processor sharing).  A number (the default, 2e7) gives the next chunks
+    to the runs with the least remaining work (step_cost x steps left) first,
+    as many as fit that per-step cost budget (at least min_inflight), so
+    short runs finish early and long ones use the slack."""
+
+    def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=128, tuner=None,
+                 inflight_work=2e7, min_inflight=6, max_wait=16, device=0, batched=True, nslots=16):
         if hbm_budget is None:
             hbm_budget = device_memory(device)[0] * 0.8
         self.budget = hbm_budget
@@ -182,10 +200,14 @@ class Server:
         self.inflight_work = inflight_work
         self.min_inflight = max(1, int(min_inflight))
         self.max_wait = max_wait
+        self.device = device
+        self.batched = batched
+        self.nslots = nslots
 
     def _start(self, req, now):
         args = req.solver_args(self.precision)
-        v = self.tuner.variant(args) if self.tuner else 0
+        # batched launches use the default (staged) z-pass kernels
+        v = self.tuner.variant(args) if (self.tuner and not self.batched) else 0
         s = Solver(zpass_variant=v, **args)
         try:
             s.init_random(seed=req.seed, k0=req.k0)
@@ -217,7 +239,125 @@ class Server:
         if a.get("solver") is not None:
             a["solver"].close()
 
+    @staticmethod
+    def _stats(done, wall, steps_done):
+        ok = [r for r in done if not r.error]
+        lat = np.array([r.finished - r.arrival for r in ok])
+        return {
+            "requests": len(done),
+            "failed": len(done) - len(ok),
+            "wall_s": wall,
+            "runs_per_s": len(ok) / wall,
+            "steps_per_s": steps_done / wall,
+            "grid_point_steps_per_s": sum(fft_points(r) * r.steps for r in ok) / wall,
+            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else math.nan,
+            "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else math.nan,
+            "latency_mean_s": float(lat.mean()) if len(lat) else math.nan,
+        }
+
     def run(self, requests, keep_state=False):
+        """Serves the requests (arrival times relative to the call); returns
+        (stats, the requests in completion order)."""
+        if self.batched:
+            return self._run_batched(requests, keep_state)
+        return self._run_unbatched(requests, keep_state)
+
+    def _run_batched(self, requests, keep_state):
+        pending = sorted(requests, key=lambda r: r.arrival)
+        classes = {}  # class key -> list of groups {"batch", "slots": {slot: run}, "waiting": [run], "busy"}
+        done = []
+        used, n_runs, steps_done = 0.0, 0, 0
+        t0 = time.perf_counter()
+
+        def retire(g, slot, a, failed_err=None):
+            nonlocal used, n_runs
+            g["batch"].detach(slot)
+            del g["slots"][slot]
+            if failed_err is not None:
+                self._fail(a, failed_err, time.perf_counter() - t0)
+            else:
+                self._finish(a, time.perf_counter() - t0)
+                if not keep_state:
+                    a["req"].result.pop("state", None)
+            used -= a["bytes"]
+            n_runs -= 1
+            done.append(a["req"])
+
+        try:
+            while pending or n_runs:
+                now = time.perf_counter() - t0
+                progressed = False
+                # admission by HBM footprint, in arrival order; the run is
+                # initialised now and waits for a slot of its class's batch
+                while pending and pending[0].arrival <= now and n_runs < self.max_active:
+                    r = pending[0]
+                    need = device_bytes(**r.solver_args(self.precision))
+                    if used + need > self.budget and n_runs:
+                        break
+                    pending.pop(0)
+                    try:
+                        a = self._start(r, now)
+                    except SpnsError as e:
+                        self._fail({"req": r}, e, now)
+                        done.append(r)
+                        continue
+                    used += a["bytes"]
+                    n_runs += 1
+                    groups = classes.setdefault(class_key(r, self.precision), [])
+                    g = next((g for g in groups if len(g["slots"]) + len(g["waiting"]) < self.nslots), None)
+                    if g is None:
+                        g = {"batch": Batch(self.nslots, self.chunk, self.device), "slots": {}, "waiting": [], "busy": False}
+                        groups.append(g)
+                    g["waiting"].append(a)
+                    progressed = True
+                # every batch between chunks: retire finished or diverged
+                # runs, seat waiting ones, launch the next chunk
+                for groups in classes.values():
+                    for g in groups:
+                        if g["busy"]:
+                            if not g["batch"].query():
+                                continue
+                            g["busy"] = False
+                        for slot, a in list(g["slots"].items()):
+                            try:
+                                a["solver"].query()  # raises SPNS_ENONFINITE if the run diverged
+                            except SpnsError as e:
+                                retire(g, slot, a, e)
+                                progressed = True
+                                continue
+                            if a["done"] >= a["req"].steps:
+                                retire(g, slot, a)
+                                progressed = True
+                        while g["waiting"]:
+                            free = g["batch"].free_slots()
+                            if not free:
+                                break
+                            a = g["waiting"].pop(0)
+                            g["batch"].attach(free[0], a["solver"])
+                            g["slots"][free[0]] = a
+                        plan = [0] * self.nslots
+                        for slot, a in g["slots"].items():
+                            plan[slot] = min(self.chunk, a["req"].steps - a["done"])
+                        if any(plan):
+                            g["batch"].step(plan)
+                            g["busy"] = True
+                            for slot, a in g["slots"].items():
+                                a["done"] += plan[slot]
+                                steps_done += plan[slot]
+                            progressed = True
+                if not progressed:
+                    if not n_runs and pending:
+                        time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                    else:
+                        time.sleep(2e-4)
+        finally:
+            for groups in classes.values():
+                for g in groups:
+                    g["batch"].close()
+        wall = time.perf_counter() - t0
+        return self._stats(done, wall, steps_done), done
+
+    def _run_unbatched(self, requests, keep_state):
         pending = sorted(requests, key=lambda r: r.arrival)
         if self.tuner:
             self.tuner.prepare([r.solver_args(self.precision) for r in pending])
@@ -286,18 +426,7 @@ class Server:
             if not progressed:
                 time.sleep(2e-4)
         wall = time.perf_counter() - t0
-        ok = [r for r in done if not r.error]
-        lat = np.array([r.finished - r.arrival for r in ok])
-        return {
-            "requests": len(done),
-            "failed": len(done) - len(ok),
-            "wall_s": wall,
-            "runs_per_s": len(ok) / wall,
-            "steps_per_s": steps_done / wall,
-            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else math.nan,
-            "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else math.nan,
-            "latency_mean_s": float(lat.mean()) if len(lat) else math.nan,
-        }, done
+        return self._stats(done, wall, steps_done), done
 
 
 def run_alone(req, precision=PREC_FP32_SPLIT, zpass_variant=0):

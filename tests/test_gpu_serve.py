@@ -1,16 +1,19 @@
-"""Serving loop: continuous batching of mixed runs; each result is the same
+"""Serving loop: continuous batching of mixed runs, batched (one launch per
+pass over the runs of a shape class) and per-run; each result is the same
 alone and in the mix (bit for bit: runs share no state and the kernels are
-deterministic), and admission respects the HBM budget."""
+deterministic), a diverging run is retired alone, and admission respects the
+HBM budget."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def test_mix_equals_alone():
+@pytest.mark.parametrize("batched", [True, False])
+def test_mix_equals_alone(batched):
     from spns_amd import serve
     reqs = serve.draw_requests(8, rate=200.0, seed=3, steps=(20, 60))
-    srv = serve.Server(hbm_budget=40e9, chunk=7)
+    srv = serve.Server(hbm_budget=40e9, chunk=7, batched=batched)
     stats, done = srv.run(reqs, keep_state=True)
     assert stats["requests"] == 8
     for r in done[:4]:
@@ -35,7 +38,8 @@ def test_admission_respects_budget():
     assert stats["latency_p99_s"] >= stats["latency_p50_s"] > 0
 
 
-def test_diverging_run_is_isolated():
+@pytest.mark.parametrize("batched", [True, False])
+def test_diverging_run_is_isolated(batched):
     """One request with a fixed time step far above its CFL limit overflows
     to Inf/NaN within a few steps.  The loop retires it with an error
     (SPNS_ENONFINITE from spns_query) after that chunk and the other runs
@@ -45,7 +49,7 @@ def test_diverging_run_is_isolated():
     bad = reqs[2]
     bad.grid, bad.C_Smag, bad.nu, bad.steps = (32, 32, 32), 0.0, 1e-4, 400
     bad.fixed_dt = 5.0  # the CFL limit here is about 1e-2
-    srv = serve.Server(hbm_budget=40e9, chunk=10)
+    srv = serve.Server(hbm_budget=40e9, chunk=10, batched=batched)
     stats, done = srv.run(reqs, keep_state=True)
     assert stats["requests"] == 6 and stats["failed"] == 1
     by = {r.rid: r for r in done}
