@@ -167,6 +167,17 @@ int spns_recalculate_velocity_antitransform(spns_ctx *ctx);
  * (the current field before any step).  Needs config.gradient_invariants;
  * SPNS_EUNSUPPORTED otherwise. */
 int spns_invariant_spectra(spns_ctx *ctx, int32_t nbins, double *spectra);
+/* spns_invariant_spectra without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops with a fixed footprint, harnesses that
+ * forbid hipMalloc).  The shell sums' block partials and results live in a
+ * fixed 4 MiB device array each GPU holds for the life of the process: the
+ * call needs (min(1024, nk1*nk2) + 1) * 5 * nbins doubles of it, at most
+ * 2^19 (up to 102 shells on larger cross-sections, cubic grids up to
+ * N = 116); larger grids return SPNS_EINVAL.  Same results as
+ * spns_invariant_spectra, bit for bit.  All calls on one GPU share that
+ * array: calls that overlap on one GPU (several host threads) race. */
+int spns_invariant_spectra_noalloc(spns_ctx *ctx, int32_t nbins, double *spectra);
 /* Wait for the context's stream. */
 int spns_synchronize(spns_ctx *ctx);
 /* Non-blocking: *done = 1 when every step enqueued by the last spns_step call

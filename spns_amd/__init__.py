@@ -60,6 +60,7 @@ EXPORTS = [
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
     "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_check_phys_four_energy",
     "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_device_memory", "spns_last_step_ms", "spns_invariant_spectra",
+    "spns_invariant_spectra_noalloc",
     "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
@@ -109,6 +110,7 @@ def load_library(path=LIB_PATH):
         "spns_query": (I32, [P, ctypes.POINTER(ctypes.c_int32)]),
         "spns_device_memory": (I32, [I32, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "spns_invariant_spectra": (I32, [P, I32, D]),
+        "spns_invariant_spectra_noalloc": (I32, [P, I32, D]),
         "spns_last_step_ms": (I32, [P, D]),
         "spns_profile_passes": (I32, [P, I32, D]),
         "spns_fft_create": (I32, [I32, I32, I32, I32, I32, ctypes.POINTER(P)]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -141,6 +142,46 @@ struct spns_fft {
 };
 
 // ===========================================================================
+// Fixed device scratch for spns_invariant_spectra_noalloc: the shell sums'
+// block partials and results live here instead of in memory the library
+// allocates.  Each GPU holds its own copy (4 MiB) until the process ends.
+// In host code `scratch` names a host-side stand-in, never device memory:
+// only the address from hipGetSymbolAddress (noalloc_scratch) is passed to
+// kernels and copies.  Every call on one GPU shares that GPU's array, so
+// calls that overlap on one GPU race.
+// ===========================================================================
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 19;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (spns_create calls it); nullptr if the lookup failed.
+static double* noalloc_scratch(int device, hipError_t* err)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void* p = nullptr;
+        g_scratch_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? (double*)p : nullptr;
+        if (g_scratch_err[device] != hipSuccess) (void)hipGetLastError();
+        g_scratch_done[device] = true;
+    }
+    *err = g_scratch_err[device];
+    return g_scratch_ptr[device];
+}
+
+// ===========================================================================
 // Solver context
 // ===========================================================================
 struct spns_ctx {
@@ -165,6 +206,8 @@ struct spns_ctx {
     int n_x1 = 0, nfwd = 3;
     bool keep_prev = false;           // ukprev holds the start-of-step state
     double* invspec = nullptr;        // [5][nbins] invariant spectra, then the block partials
+    double* scratch_dev = nullptr;    // this device's fixed scratch (spns_invariant_spectra_noalloc)
+    hipError_t scratch_err = hipSuccess;
     void *twx = nullptr, *twy = nullptr, *twz = nullptr;
     DevScalars* scal = nullptr;
     double* hist_partial = nullptr;
@@ -975,6 +1018,7 @@ int spns_create(const spns_config* cfg, spns_ctx** out)
     spns_ctx* c = new spns_ctx();
     try {
         c->init(*cfg);
+        c->scratch_dev = noalloc_scratch(c->cfg.device, &c->scratch_err);  // device is current after init
     } catch (const Error& e) {
         g_create_error = e.what();
         c->release();
@@ -1407,6 +1451,25 @@ int spns_invariant_spectra(spns_ctx* ctx, int32_t nbins, double* spectra)
     ctx->invariant_spectra(h.data());
     for (int f = 0; f < 5; f++)
         for (int K = 0; K < ctx->nbins; K++) spectra[(size_t)f * nbins + K] = h[(size_t)f * ctx->nbins + K];
+    API_END(ctx)
+}
+
+int spns_invariant_spectra_noalloc(spns_ctx* ctx, int32_t nbins, double* spectra)
+{
+    API_BEGIN(ctx)
+    if (!spectra || nbins < ctx->nbins) throw Error(SPNS_EINVAL, "spectra must hold 5 x spns_nbins values");
+    const size_t need = ctx->inv_spectra_scratch();
+    if (need > kNoAllocScratchDoubles)
+        throw Error(SPNS_EINVAL, "invariant spectra of this grid need " + std::to_string(need) +
+                                     " doubles of scratch; the fixed scratch holds " +
+                                     std::to_string(kNoAllocScratchDoubles) + " (use spns_invariant_spectra)");
+    if (!ctx->scratch_dev)
+        throw Error(SPNS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(ctx->scratch_err));
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    ctx->invariant_spectra_dev(ctx->scratch_dev);
+    HIPCHK(hipMemcpy2DAsync(spectra, sizeof(double) * nbins, ctx->scratch_dev, sizeof(double) * ctx->nbins,
+                            sizeof(double) * ctx->nbins, 5, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
     API_END(ctx)
 }
 
