@@ -83,6 +83,14 @@ typedef struct spns_scalars {
 
 /* Replaces HIT::HIT + HIT_init allocation (src/HIT_LES.h:23-84, src/HIT_LES.cpp:770-1092). */
 int spns_create(const spns_config *cfg, spns_ctx **out);
+/* Reuses a context for another run of the same shape class (no reference
+ * counterpart; the serving loop keeps finished runs' contexts instead of
+ * freeing and allocating device memory, which synchronises the device): the
+ * physics of cfg (nu, C_Smag, C_At, rotation, fixed_dt, forcing, delta) for
+ * the same grid, precision, model and options; state, history, time and
+ * iteration count are cleared.  Then set an initial field and call
+ * spns_finish_init.  SPNS_EINVAL if anything but the physics differs. */
+int spns_reconfigure(spns_ctx *ctx, const spns_config *cfg);
 /* Replaces HIT::~HIT / HIT_destroy (src/HIT_LES.cpp:1094-1130). */
 int spns_destroy(spns_ctx *ctx);
 /* Device memory a context with this configuration allocates (bytes), for
@@ -101,6 +109,10 @@ const char *spns_last_error(const spns_ctx *ctx);
 /* State in the compact layout above: 3*nmodes complex (6*nmodes doubles). */
 int spns_set_state(spns_ctx *ctx, const double *state);
 int spns_get_state(spns_ctx *ctx, double *state);
+/* The state's i2 planes [i2_first, i2_first + n_i2) only: [c][n_i2][nk1][nk3]
+ * complex double (the compact layout above restricted to those planes), for
+ * checks on grids whose whole state does not fit the host comfortably. */
+int spns_get_state_planes(spns_ctx *ctx, int32_t i2_first, int32_t n_i2, double *out);
 /* R history (R of the last step), compact layout. */
 int spns_get_rhist(spns_ctx *ctx, double *rhist);
 /* HIT::Input_Real_Field (src/HIT_LES.cpp:542-599, 738-763): u,v,w are
@@ -187,6 +199,11 @@ int spns_synchronize(spns_ctx *ctx);
  * its runs with it and retires a diverged run alone, where the reference's
  * crash() would abort every rank, src/util.cpp:93-116). */
 int spns_query(spns_ctx *ctx, int32_t *done);
+/* Measured practical HBM ceiling: a streaming copy kernel (16-byte
+ * non-temporal loads and stores) between two device buffers of `bytes`
+ * each, reps times; *gb_per_s = bytes read + written per second (1e9 B).
+ * No reference counterpart (measurement). */
+int spns_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double *gb_per_s);
 /* Free and total device memory of a HIP device (hipMemGetInfo), for
  * admission by HBM footprint.  No reference counterpart. */
 int spns_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes);

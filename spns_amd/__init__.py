@@ -53,13 +53,13 @@ class Scalars(ctypes.Structure):
 
 
 EXPORTS = [
-    "spns_create", "spns_destroy", "spns_device_bytes", "spns_nmodes", "spns_nbins", "spns_last_rad", "spns_last_error",
-    "spns_set_state", "spns_get_state", "spns_get_rhist", "spns_init_from_physical", "spns_init_k41",
+    "spns_create", "spns_reconfigure", "spns_destroy", "spns_device_bytes", "spns_nmodes", "spns_nbins", "spns_last_rad", "spns_last_error",
+    "spns_set_state", "spns_get_state", "spns_get_state_planes", "spns_get_rhist", "spns_init_from_physical", "spns_init_k41",
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
     "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
     "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_check_phys_four_energy",
-    "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_device_memory", "spns_last_step_ms", "spns_invariant_spectra",
+    "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_device_memory", "spns_copy_bandwidth", "spns_last_step_ms", "spns_invariant_spectra",
     "spns_invariant_spectra_noalloc",
     "spns_profile_passes",
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
@@ -81,6 +81,7 @@ def load_library(path=LIB_PATH):
     sig = {
         "spns_create": (I32, [ctypes.POINTER(Config), ctypes.POINTER(P)]),
         "spns_destroy": (I32, [P]),
+        "spns_reconfigure": (I32, [P, ctypes.POINTER(Config)]),
         "spns_device_bytes": (I64, [ctypes.POINTER(Config)]),
         "spns_nmodes": (I64, [P]),
         "spns_nbins": (I32, [P]),
@@ -109,6 +110,8 @@ def load_library(path=LIB_PATH):
         "spns_synchronize": (I32, [P]),
         "spns_query": (I32, [P, ctypes.POINTER(ctypes.c_int32)]),
         "spns_device_memory": (I32, [I32, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "spns_copy_bandwidth": (I32, [I32, I64, I32, D]),
+        "spns_get_state_planes": (I32, [P, I32, I32, D]),
         "spns_invariant_spectra": (I32, [P, I32, D]),
         "spns_invariant_spectra_noalloc": (I32, [P, I32, D]),
         "spns_last_step_ms": (I32, [P, D]),
@@ -180,6 +183,17 @@ def device_bytes(**kw):
     return int(load_library().spns_device_bytes(ctypes.byref(make_config(**kw))))
 
 
+def copy_bandwidth(nbytes=4 << 30, reps=10, device=0):
+    """Measured streaming-copy bandwidth (GB/s of bytes read + written) of a
+    hand-written copy kernel between two buffers of nbytes (spns_copy_bandwidth)."""
+    L = load_library()
+    g = ctypes.c_double()
+    rc = L.spns_copy_bandwidth(device, nbytes, reps, ctypes.byref(g))
+    if rc:
+        raise SpnsError(rc, L.spns_last_error(None).decode())
+    return g.value
+
+
 def device_memory(device=0):
     """(free, total) bytes of a HIP device (spns_device_memory)."""
     L = load_library()
@@ -201,6 +215,11 @@ class Solver:
                  force_kmin=1, force_kmax=2, device=0, use_graph=True, zpass_variant=0, gradient_invariants=False,
                  keep_step_start=False):
         self.L = load_library()
+        self.args = dict(Nx=Nx, Ny=Ny, Nz=Nz, Mx=Mx, My=My, Mz=Mz, nu=nu, C_Smag=C_Smag, C_At=C_At, Lx=Lx, Ly=Ly, Lz=Lz,
+                         omega=omega, isNotCubic=isNotCubic, isSelfAdaptive=isSelfAdaptive, isCCC=isCCC,
+                         precision=precision, delta=delta, fixed_dt=fixed_dt, force_eps=force_eps, force_kmin=force_kmin,
+                         force_kmax=force_kmax, device=device, use_graph=use_graph, zpass_variant=zpass_variant,
+                         gradient_invariants=gradient_invariants, keep_step_start=keep_step_start)
         c = make_config(Nx, Ny, Nz, Mx, My, Mz, nu, C_Smag, C_At, Lx, Ly, Lz, omega, isNotCubic, isSelfAdaptive,
                         isCCC, precision, delta, fixed_dt, force_eps, force_kmin, force_kmax, device, use_graph,
                         zpass_variant, gradient_invariants, keep_step_start)
@@ -223,6 +242,18 @@ class Solver:
         if rc:
             raise SpnsError(rc, self.L.spns_last_error(self.h).decode())
 
+    def reconfigure(self, **kw):
+        """Reuses this run's device buffers for another run of the same shape
+        class (spns_reconfigure): new physics (nu, C_Smag, C_At, omega,
+        fixed_dt, forcing, delta), state and history cleared.  The grid,
+        precision and options must stay; set an initial field next."""
+        args = dict(self.args)
+        args.update(kw)
+        c = make_config(**args)
+        self._chk(self.L.spns_reconfigure(self.h, ctypes.byref(c)))
+        self.args = args
+        self.cfg = c
+
     def close(self):
         if getattr(self, "h", None):
             self.L.spns_destroy(self.h)
@@ -242,6 +273,12 @@ class Solver:
     def get_state(self):
         out = np.zeros((3, self.nmodes), dtype=np.complex128)
         self._chk(self.L.spns_get_state(self.h, _dp(out.view(np.float64))))
+        return out
+
+    def get_state_planes(self, i2_first, n_i2):
+        """State on the i2 planes [i2_first, i2_first + n_i2): [3][n_i2][nk1][nk3]."""
+        out = np.zeros((3, n_i2, self.nk[1], self.nk[2]), dtype=np.complex128)
+        self._chk(self.L.spns_get_state_planes(self.h, i2_first, n_i2, _dp(out.view(np.float64))))
         return out
 
     def get_rhist(self):

@@ -349,6 +349,49 @@ struct spns_ctx {
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    // Takes the physics of c (viscosity, C_s, CFL constant, rotation, fixed
+    // step, forcing, filter width) for a run of the same shape class and
+    // clears state, history, time and iteration count; the device buffers
+    // stay.  The caller sets an initial field and calls spns_finish_init.
+    void reconfigure(const spns_config& c)
+    {
+        const int cLx = c.Lx > 0 ? c.Lx : 1, cLy = c.Ly > 0 ? c.Ly : 1, cLz = c.Lz > 0 ? c.Lz : 1;
+        const int cMx = c.Mx > 0 ? c.Mx : (3 * c.Nx) / 2, cMy = c.My > 0 ? c.My : (3 * c.Ny) / 2,
+                  cMz = c.Mz > 0 ? c.Mz : (3 * c.Nz) / 2;
+        const bool same = c.Nx == Nx && c.Ny == Ny && c.Nz == Nz && cMx == Mx && cMy == My && cMz == Mz &&
+                          c.precision == cfg.precision && (c.C_Smag > 0.0) == les && cLx == cfg.Lx && cLy == cfg.Ly &&
+                          cLz == cfg.Lz && c.isNotCubic == cfg.isNotCubic && c.isSelfAdaptive == cfg.isSelfAdaptive &&
+                          c.gradient_invariants == cfg.gradient_invariants && c.keep_step_start == cfg.keep_step_start &&
+                          c.zpass_variant == cfg.zpass_variant && c.device == cfg.device;
+        if (!same) throw Error(SPNS_EINVAL, "reconfigure: the grid, precision, model and buffers must stay the same");
+        HIPCHK(hipStreamSynchronize(st));
+        const spns_config old = cfg;
+        cfg = c;
+        cfg.Lx = cLx; cfg.Ly = cLy; cfg.Lz = cLz;
+        cfg.Mx = Mx; cfg.My = My; cfg.Mz = Mz;
+        delta = cfg.delta > 0 ? cfg.delta
+                              : 2.0 * kPi / std::cbrt((double)(cfg.Lx * cfg.Ly * cfg.Lz) * ActualMx * ActualMy * ActualMz);
+        ct = cfg.C_Smag * cfg.C_Smag * delta * delta;  // src/HIT_LES.cpp:132
+        rot = (std::fabs(cfg.omega_x) > 0 || std::fabs(cfg.omega_y) > 0 || std::fabs(cfg.omega_z) > 0);
+        (void)old;
+        if (graph) {
+            hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+        graph_ready = false;
+        const size_t sb = sizeof(cx<double>) * (size_t)nmodes_p;
+        for (int c3 = 0; c3 < 3; c3++) {
+            HIPCHK(hipMemsetAsync(uk[c3], 0, sb, st));
+            HIPCHK(hipMemsetAsync(R[c3], 0, sb, st));
+            if (keep_prev) HIPCHK(hipMemsetAsync(ukprev[c3], 0, sb, st));
+        }
+        DevScalars s{};
+        s.nu = cfg.nu;
+        s.kappa = 0.5;  // src/HIT_LES.cpp:1053
+        write_scalars(s);
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
     void alloc(void** p, size_t bytes)
     {
         hipError_t e = hipMalloc(p, bytes);
@@ -1034,6 +1077,15 @@ int spns_create(const spns_config* cfg, spns_ctx** out)
     return SPNS_OK;
 }
 
+int spns_reconfigure(spns_ctx* ctx, const spns_config* cfg)
+{
+    API_BEGIN(ctx)
+    if (!cfg) throw Error(SPNS_EINVAL, "null config");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    ctx->reconfigure(*cfg);
+    API_END(ctx)
+}
+
 int spns_destroy(spns_ctx* ctx)
 {
     if (!ctx) return SPNS_EINVAL;
@@ -1079,6 +1131,20 @@ int spns_get_state(spns_ctx* ctx, double* state)
     if (!state) throw Error(SPNS_EINVAL, "null output");
     auto h = ctx->download_state(ctx->uk);
     std::memcpy(state, h.data(), sizeof(double) * 6 * ctx->nmodes);
+    API_END(ctx)
+}
+
+int spns_get_state_planes(spns_ctx* ctx, int32_t i2_first, int32_t n_i2, double* out)
+{
+    API_BEGIN(ctx)
+    if (!out || i2_first < 0 || n_i2 < 0 || (int64_t)i2_first + n_i2 > ctx->nk2)
+        throw Error(SPNS_EINVAL, "i2 range outside the compact box");
+    const size_t plane = (size_t)ctx->nk1 * ctx->nk3;  // modes per i2 plane (compact)
+    for (int c = 0; c < 3; c++)
+        HIPCHK(hipMemcpy2DAsync(out + 2 * (size_t)c * n_i2 * plane, sizeof(cx<double>) * ctx->nk3,
+                                ctx->uk[c] + (size_t)i2_first * ctx->nk1 * ctx->nk3p, sizeof(cx<double>) * ctx->nk3p,
+                                sizeof(cx<double>) * ctx->nk3, (size_t)n_i2 * ctx->nk1, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
     API_END(ctx)
 }
 
@@ -1427,6 +1493,44 @@ int spns_query(spns_ctx* ctx, int32_t* done)
             throw Error(SPNS_ENONFINITE, "non-finite velocity field after step " + std::to_string(s.iter));
     }
     API_END(ctx)
+}
+
+int spns_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gb_per_s)
+{
+    if (!gb_per_s || bytes <= 0 || bytes % 16 || reps < 1) return SPNS_EINVAL;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SPNS_OK;
+    try {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipMalloc(&a, bytes));
+        HIPCHK(hipMalloc(&b, bytes));
+        HIPCHK(hipMemsetAsync(a, 0, bytes, st));
+        int ncu = 0;
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        const int blocks = ncu * 16;
+        HIPCHK(launch_stream_copy(a, b, bytes, blocks, st));  // warm-up
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++) HIPCHK(launch_stream_copy((r & 1) ? b : a, (r & 1) ? a : b, bytes, blocks, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *gb_per_s = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;  // bytes read + written
+    } catch (const Error& e) {
+        g_create_error = e.what();
+        rc = e.code;
+    }
+    if (st) hipStreamSynchronize(st);
+    hipFree(a); hipFree(b);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    return rc;
 }
 
 int spns_device_memory(int32_t device, int64_t* free_bytes, int64_t* total_bytes)

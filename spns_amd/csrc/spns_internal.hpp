@@ -191,6 +191,7 @@ hipError_t launch_finalize(const FinArgs& a, hipStream_t st, const BatchLaunch* 
 hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* out, hipStream_t st);
 hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_t st, const BatchLaunch* bl = nullptr);
 hipError_t launch_set_left(int* left, const SlotSteps& s, int nslots, hipStream_t st);
+hipError_t launch_stream_copy(const void* in, void* out, int64_t bytes, int blocks, hipStream_t st);
 hipError_t launch_forcing(const double* ek, const double* forced, double* ft, int last_eff, const ShellArgs& a, hipStream_t st);
 hipError_t launch_ccc(const ShellArgs& a, hipStream_t st);
 hipError_t launch_normalize(int prec, const NormArgs& a, hipStream_t st);

@@ -1821,6 +1821,30 @@ static __global__ void k_clear_b(const ClearArgs* __restrict__ arr, const int* _
     if (threadIdx.x == 0) { *arr[blockIdx.y].a = 0ull; *arr[blockIdx.y].b = 0ull; }
 }
 
+// Streaming copy (the measured practical HBM ceiling beside the data-sheet
+// peak): each thread moves U 16-byte vectors per iteration, all loads
+// issued before the stores, non-temporal so the copy does not keep lines in
+// the caches; XCD-interleaved block order is irrelevant for a linear sweep.
+typedef float vf4 __attribute__((ext_vector_type(4)));
+template <int U>
+__global__ __launch_bounds__(256) void k_stream_copy(const vf4* __restrict__ in, vf4* __restrict__ out, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * 256 * U;
+    for (int64_t base = (int64_t)blockIdx.x * 256 * U + threadIdx.x; base < n; base += stride) {
+        vf4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = base + (int64_t)u * 256;
+            if (i < n) v[u] = __builtin_nontemporal_load(in + i);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = base + (int64_t)u * 256;
+            if (i < n) __builtin_nontemporal_store(v[u], out + i);
+        }
+    }
+}
+
 // Steps-left counters of a batch for its next chunk, from the kernel's
 // arguments (so the host may reuse its copy right after the launch).
 static __global__ void k_set_left(int* left, SlotSteps s, int nslots)
@@ -2492,6 +2516,13 @@ hipError_t launch_clear(unsigned long long* a, unsigned long long* b, hipStream_
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(const void* in, void* out, int64_t bytes, int blocks, hipStream_t st)
+{
+    if (bytes % 16 || blocks < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stream_copy<4>), dim3(blocks), dim3(256), 0, st, (const vf4*)in, (vf4*)out, bytes / 16);
     return hipGetLastError();
 }
 

@@ -189,7 +189,8 @@ class Server:
     short runs finish early and long ones use the slack."""
 
     def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=128, tuner=None,
-                 inflight_work=2e7, min_inflight=6, max_wait=16, device=0, batched=True, nslots=16):
+                 inflight_work=2e7, min_inflight=6, max_wait=16, device=0, batched=True, nslots=16,
+                 keep_contexts=True):
         if hbm_budget is None:
             hbm_budget = device_memory(device)[0] * 0.8
         self.budget = hbm_budget
@@ -203,20 +204,69 @@ class Server:
         self.device = device
         self.batched = batched
         self.nslots = nslots
+        self.keep_contexts = keep_contexts
+        self.used = 0.0
+        self.reused = 0
+        self._kept = {}
+
+    # Finished runs' contexts are kept per shape class and reused by later
+    # runs of that class (spns_reconfigure): freeing device memory waits for
+    # all work on the device, which would stall the host, and every batch
+    # with it, at each retirement.  Kept contexts count against the HBM
+    # budget and are freed only when a new run needs their memory.
+    def _pool_key(self, args, v):
+        return (args["Nx"], args["Ny"], args["Nz"], args["precision"], args["C_Smag"] > 0.0, v)
+
+    def _evict(self, need):
+        """Frees kept contexts (oldest first) until need bytes fit the budget."""
+        while self.used + need > self.budget and self._kept:
+            key = next(iter(self._kept))
+            s, nbytes = self._kept[key].pop(0)
+            if not self._kept[key]:
+                del self._kept[key]
+            s.close()
+            self.used -= nbytes
 
     def _start(self, req, now):
         args = req.solver_args(self.precision)
         # batched launches use the default (staged) z-pass kernels
         v = self.tuner.variant(args) if (self.tuner and not self.batched) else 0
-        s = Solver(zpass_variant=v, **args)
+        key = self._pool_key(args, v)
+        nbytes = device_bytes(zpass_variant=v, **args)
+        kept = self._kept.get(key)
+        if kept:
+            s, nbytes = kept.pop(0)
+            if not kept:
+                del self._kept[key]
+            self.reused += 1
+            try:
+                s.reconfigure(**args)
+            except SpnsError:
+                s.close()
+                self.used -= nbytes
+                raise
+        else:
+            s = Solver(zpass_variant=v, **args)
+            self.used += nbytes
         try:
             s.init_random(seed=req.seed, k0=req.k0)
             s.finish_init(False)
         except SpnsError:
             s.close()
+            self.used -= nbytes
             raise
         req.admitted = now
-        return {"req": req, "solver": s, "done": 0, "waited": 0, "bytes": device_bytes(zpass_variant=v, **args)}
+        return {"req": req, "solver": s, "done": 0, "waited": 0, "bytes": nbytes, "key": key}
+
+    def _release(self, a, keep):
+        s = a.get("solver")
+        if s is None:
+            return
+        if keep and self.keep_contexts:
+            self._kept.setdefault(a["key"], []).append((s, a["bytes"]))
+        else:
+            s.close()
+            self.used -= a["bytes"]
 
     def _finish(self, a, now):
         s, req = a["solver"], a["req"]
@@ -228,16 +278,21 @@ class Server:
             self._fail(a, e, now)
             return
         req.finished = now
-        s.close()
+        self._release(a, keep=True)
 
-    @staticmethod
-    def _fail(a, err, now):
+    def _fail(self, a, err, now):
         req = a["req"]
         req.error = str(err)
         req.result = {"error_code": getattr(err, "code", -1), "steps": a.get("done", 0)}
         req.finished = now
-        if a.get("solver") is not None:
-            a["solver"].close()
+        self._release(a, keep=False)  # a diverged run's buffers are not reused
+
+    def _close_kept(self):
+        for lst in self._kept.values():
+            for s, nbytes in lst:
+                s.close()
+                self.used -= nbytes
+        self._kept = {}
 
     @staticmethod
     def _stats(done, wall, steps_done):
@@ -258,19 +313,22 @@ class Server:
     def run(self, requests, keep_state=False):
         """Serves the requests (arrival times relative to the call); returns
         (stats, the requests in completion order)."""
-        if self.batched:
-            return self._run_batched(requests, keep_state)
-        return self._run_unbatched(requests, keep_state)
+        self.reused = 0
+        stats, done = (self._run_batched if self.batched else self._run_unbatched)(requests, keep_state)
+        stats["contexts_reused"] = self.reused
+        return stats, done
 
     def _run_batched(self, requests, keep_state):
         pending = sorted(requests, key=lambda r: r.arrival)
         classes = {}  # class key -> list of groups {"batch", "slots": {slot: run}, "waiting": [run], "busy"}
         done = []
-        used, n_runs, steps_done = 0.0, 0, 0
+        n_runs, steps_done = 0, 0
+        host = {"start": 0.0, "retire": 0.0, "attach": 0.0, "launch": 0.0, "poll": 0.0, "idle": 0.0}
+        clock = time.perf_counter
         t0 = time.perf_counter()
 
         def retire(g, slot, a, failed_err=None):
-            nonlocal used, n_runs
+            nonlocal n_runs
             g["batch"].detach(slot)
             del g["slots"][slot]
             if failed_err is not None:
@@ -279,7 +337,6 @@ class Server:
                 self._finish(a, time.perf_counter() - t0)
                 if not keep_state:
                     a["req"].result.pop("state", None)
-            used -= a["bytes"]
             n_runs -= 1
             done.append(a["req"])
 
@@ -291,17 +348,21 @@ class Server:
                 # initialised now and waits for a slot of its class's batch
                 while pending and pending[0].arrival <= now and n_runs < self.max_active:
                     r = pending[0]
-                    need = device_bytes(**r.solver_args(self.precision))
-                    if used + need > self.budget and n_runs:
+                    args = r.solver_args(self.precision)
+                    need = 0.0 if self._kept.get(self._pool_key(args, 0)) else device_bytes(**args)
+                    self._evict(need)
+                    if self.used + need > self.budget and n_runs:
                         break
                     pending.pop(0)
+                    tc = clock()
                     try:
                         a = self._start(r, now)
                     except SpnsError as e:
                         self._fail({"req": r}, e, now)
                         done.append(r)
                         continue
-                    used += a["bytes"]
+                    finally:
+                        host["start"] += clock() - tc
                     n_runs += 1
                     groups = classes.setdefault(class_key(r, self.precision), [])
                     g = next((g for g in groups if len(g["slots"]) + len(g["waiting"]) < self.nslots), None)
@@ -315,9 +376,13 @@ class Server:
                 for groups in classes.values():
                     for g in groups:
                         if g["busy"]:
-                            if not g["batch"].query():
+                            tc = clock()
+                            busy = not g["batch"].query()
+                            host["poll"] += clock() - tc
+                            if busy:
                                 continue
                             g["busy"] = False
+                        tc = clock()
                         for slot, a in list(g["slots"].items()):
                             try:
                                 a["solver"].query()  # raises SPNS_ENONFINITE if the run diverged
@@ -328,6 +393,8 @@ class Server:
                             if a["done"] >= a["req"].steps:
                                 retire(g, slot, a)
                                 progressed = True
+                        host["retire"] += clock() - tc
+                        tc = clock()
                         while g["waiting"]:
                             free = g["batch"].free_slots()
                             if not free:
@@ -335,34 +402,41 @@ class Server:
                             a = g["waiting"].pop(0)
                             g["batch"].attach(free[0], a["solver"])
                             g["slots"][free[0]] = a
+                        host["attach"] += clock() - tc
                         plan = [0] * self.nslots
                         for slot, a in g["slots"].items():
                             plan[slot] = min(self.chunk, a["req"].steps - a["done"])
                         if any(plan):
+                            tc = clock()
                             g["batch"].step(plan)
+                            host["launch"] += clock() - tc
                             g["busy"] = True
                             for slot, a in g["slots"].items():
                                 a["done"] += plan[slot]
                                 steps_done += plan[slot]
                             progressed = True
                 if not progressed:
+                    tc = clock()
                     if not n_runs and pending:
                         time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                     else:
                         time.sleep(2e-4)
+                    host["idle"] += clock() - tc
         finally:
             for groups in classes.values():
                 for g in groups:
                     g["batch"].close()
         wall = time.perf_counter() - t0
-        return self._stats(done, wall, steps_done), done
+        self._close_kept()
+        stats = self._stats(done, wall, steps_done)
+        stats["host_s"] = {k: round(v, 3) for k, v in host.items()}
+        return stats, done
 
     def _run_unbatched(self, requests, keep_state):
         pending = sorted(requests, key=lambda r: r.arrival)
         if self.tuner:
             self.tuner.prepare([r.solver_args(self.precision) for r in pending])
         active, done = [], []
-        used = 0.0
         t0 = time.perf_counter()
         steps_done = 0
         while pending or active:
@@ -370,8 +444,11 @@ class Server:
             # admission by HBM footprint, in arrival order
             while pending and pending[0].arrival <= now and len(active) < self.max_active:
                 r = pending[0]
-                need = device_bytes(**r.solver_args(self.precision))
-                if used + need > self.budget and active:
+                args = r.solver_args(self.precision)
+                v = self.tuner.variant(args) if self.tuner else 0
+                need = 0.0 if self._kept.get(self._pool_key(args, v)) else device_bytes(zpass_variant=v, **args)
+                self._evict(need)
+                if self.used + need > self.budget and active:
                     break
                 pending.pop(0)
                 try:
@@ -380,7 +457,6 @@ class Server:
                     self._fail({"req": r}, e, now)
                     done.append(r)
                     continue
-                used += a["bytes"]
                 active.append(a)
             if not active:
                 if pending:
@@ -399,7 +475,6 @@ class Server:
                         continue
                 except SpnsError as e:  # diverged (or a device error) in its last chunk
                     self._fail(a, e, time.perf_counter() - t0)
-                    used -= a["bytes"]
                     done.append(a["req"])
                     progressed = True
                     continue
@@ -407,7 +482,6 @@ class Server:
                     self._finish(a, time.perf_counter() - t0)
                     if not keep_state:
                         a["req"].result.pop("state", None)
-                    used -= a["bytes"]
                     done.append(a["req"])
                     progressed = True
                     continue
@@ -426,6 +500,7 @@ class Server:
             if not progressed:
                 time.sleep(2e-4)
         wall = time.perf_counter() - t0
+        self._close_kept()
         return self._stats(done, wall, steps_done), done
 
 

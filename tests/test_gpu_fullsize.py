@@ -104,11 +104,49 @@ def test_512_fp64_and_fp32_split():
     assert abs(sc32["time"] - sc64["time"]) <= 1e-6 * sc64["time"]
 
 
-def test_1024_fp32_split():
+def plane_checks(s, N, sample):
+    """Divergence and Hermitian-plane defects on the sampled i2 planes and
+    their mirrors only (spns_get_state_planes: the whole 1024^3 state is
+    26 GB on the host)."""
+    nk1 = nk2 = 2 * (N // 2) + 1
+    nk3 = N // 2 + 1
+    k1 = wavenumbers(nk1)[:, None]
+    k3 = np.arange(nk3, dtype=np.float64)[None, :]
+    k2all = wavenumbers(nk2)
+    num = den = herm_num = herm_den = 0.0
+    for i2 in sample:
+        j2 = (nk2 - i2) % nk2
+        p = s.get_state_planes(i2, 1)[:, 0]
+        q = s.get_state_planes(j2, 1)[:, 0]
+        d = k1 * p[0] + k2all[i2] * p[1] + k3 * p[2]
+        kk = np.sqrt(k1 ** 2 + k2all[i2] ** 2 + k3 ** 2)
+        mag = np.sqrt(np.sum(np.abs(p) ** 2, axis=0))
+        num += float(np.sum(np.abs(d)))
+        den += float(np.sum(kk * mag))
+        # u(-k1, -k2, 0) = conj u(k1, k2, 0)
+        j1 = (nk1 - np.arange(nk1)) % nk1
+        herm_num += float(np.sum(np.abs(p[:, :, 0] - np.conj(q[:, j1, 0])) ** 2))
+        herm_den += float(np.sum(np.abs(p[:, :, 0]) ** 2))
+    return num / den, np.sqrt(herm_num / herm_den)
+
+
+def test_1024_fp32_split_200_steps():
+    """The 1024^3 forced DNS (nu = 8e-5) over its 200 steps, FP32-split."""
     import spns_amd
-    s, sc, ek, dts = run(1024, spns_amd.PREC_FP32_SPLIT, 3)
-    # divergence on every 32nd i2 plane (the full state is 26 GB on the host)
-    common_checks(s, sc, ek, dts, 1024, 1e-13, 1e-6, planes=slice(None, None, 32))
-    # forcing holds the low shells: E stays near its initial 1.5 over 3 steps
-    assert 1.3 < sc["ek"] < 1.7
+    N = 1024
+    s, sc, ek, dts = run(N, spns_amd.PREC_FP32_SPLIT, 200)
+    for k in ("time", "dt", "ek", "om", "maxvel", "skewness"):
+        assert np.isfinite(sc[k]), k
+    assert abs(ek.sum() - sc["ek"]) <= 1e-12 * sc["ek"]
+    assert abs(sc["time"] - sum(dts)) <= 1e-12 * sc["time"]  # time += the dt computed after each step
+    sample = [0, 1, 2, 3, 5, 8, 13, 64, 200, 511, 512]
+    div, herm = plane_checks(s, N, sample)
+    print("N=1024 after 200 steps: |k.u|/|k||u| %.2e, Hermitian defect %.2e, E %.6f, eps %.6f, skewness %.4f, time %.5f" % (
+        div, herm, sc["ek"], sc["nu"] * sc["om"], sc["skewness"], sc["time"]))
+    assert div <= 1e-13
+    assert herm <= 1e-6
+    # constant power 0.1 into 1 <= K <= 2: E rises slowly, the cascade builds dissipation
+    assert 1.5 < sc["ek"] < 1.7
+    assert sc["skewness"] < 0.0
     s.close()
+

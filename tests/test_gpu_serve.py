@@ -11,12 +11,18 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("batched", [True, False])
 def test_mix_equals_alone(batched):
+    # requests 4-7 take the shape classes of 0-3 and arrive after those have
+    # finished, so they run on kept (reconfigured) contexts
     from spns_amd import serve
     reqs = serve.draw_requests(8, rate=200.0, seed=3, steps=(20, 60))
+    for i in range(4):
+        reqs[4 + i].grid, reqs[4 + i].C_Smag = reqs[i].grid, reqs[i].C_Smag
+        reqs[4 + i].arrival = 2.0 + 0.01 * i
     srv = serve.Server(hbm_budget=40e9, chunk=7, batched=batched)
     stats, done = srv.run(reqs, keep_state=True)
     assert stats["requests"] == 8
-    for r in done[:4]:
+    assert stats["contexts_reused"] >= 1  # (all four when the first runs finish before the later ones arrive)
+    for r in done:
         alone = serve.run_alone(r)
         assert np.array_equal(alone["state"], r.result["state"]), r.rid
         assert np.array_equal(alone["ek"], r.result["ek"])
@@ -69,3 +75,35 @@ def test_device_memory_query():
     free, total = device_memory(0)
     assert 0 < free <= total
     assert total > 200e9  # MI355X: 288 GB of HBM3E
+
+
+@pytest.mark.parametrize("les", [False, True])
+def test_reconfigured_context_matches_fresh(les):
+    """A finished run's context reused for another run of the same shape
+    class (spns_reconfigure, as the serving loop does) gives bit for bit what
+    a fresh context gives; anything but the physics must stay the same."""
+    import spns_amd
+    base = dict(Nx=32, C_At=0.5, precision=1)
+    a = spns_amd.Solver(nu=1e-3, C_Smag=0.17 if les else 0.0, **base)
+    a.init_random(seed=3, k0=4.0)
+    a.finish_init(False)
+    a.step(30)
+    phys = dict(nu=3e-3, C_Smag=0.12 if les else 0.0, omega=(0.0, 0.0, 5.0))
+    a.reconfigure(**phys)
+    b = spns_amd.Solver(**dict(base, **phys))
+    for s in (a, b):
+        s.init_random(seed=9, k0=4.0)
+        s.finish_init(False)
+        s.step(25)
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert np.array_equal(a.get_rhist(), b.get_rhist())
+    assert np.array_equal(a.spectrum(), b.spectrum())
+    sa, sb = a.scalars(), b.scalars()
+    assert all(sa[k] == sb[k] for k in ("time", "dt", "ek", "om", "iter"))
+    with pytest.raises(spns_amd.SpnsError) as e:
+        a.reconfigure(Nx=64)
+    assert e.value.code == spns_amd.SPNS_EINVAL
+    with pytest.raises(spns_amd.SpnsError):
+        a.reconfigure(C_Smag=0.0 if les else 0.15)  # DNS <-> LES changes the buffers
+    a.close()
+    b.close()

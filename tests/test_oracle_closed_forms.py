@@ -195,3 +195,27 @@ def test_spectrum_sums_to_energy():
     ek = o.spectrum()
     assert abs(ek.sum() - o.energy()) < 1e-14 * o.energy()
     assert ek[0] == 0.0
+
+
+def test_les_filter_width_enters_as_cs_times_delta():
+    """The explicit LES filter width (an option; the reference always uses
+    2 pi / cbrt(Mx My Mz), src/HIT_LES.h:53) enters only through
+    nu_t = (C_s Delta)^2 |S| (src/HIT_LES.cpp:130-142): the reference's
+    Delta given explicitly reproduces the default run bit for bit, the same
+    product C_s Delta reproduces it to rounding, and a larger product
+    dissipates more."""
+    d_ref = 2 * np.pi / 48  # 32^3 retained modes, FFT grid 48^3
+    kw = dict(nu=1e-4, C_At=0.5)
+    runs = {"default": O.Oracle(32, C_Smag=0.17, **kw),
+            "explicit": O.Oracle(32, C_Smag=0.17, delta=d_ref, **kw),
+            "same_product": O.Oracle(32, C_Smag=0.17 * 1.7, delta=d_ref / 1.7, **kw),
+            "larger": O.Oracle(32, C_Smag=0.17, delta=1.5 * d_ref, **kw)}
+    for o in runs.values():
+        random_state(o, 3)
+        o.finish_init(False)
+        o.step(5)
+    a = runs["default"].get_state()
+    assert np.array_equal(runs["explicit"].get_state(), a)
+    c = runs["same_product"].get_state()
+    assert np.linalg.norm(c - a) <= 1e-13 * np.linalg.norm(a)
+    assert runs["larger"].energy() < runs["default"].energy()
