@@ -241,7 +241,22 @@ int spns_batch_step(spns_batch *b, const int32_t *nsteps);
  * spns_query on a slot's context reports SPNS_ENONFINITE if it diverged. */
 int spns_batch_query(spns_batch *b, int32_t *done);
 int spns_batch_synchronize(spns_batch *b);
+/* A slot's scalars after the last spns_batch_step, from a copy the batch
+ * makes on its own stream after every chunk (no device read here); once
+ * spns_batch_query has reported that chunk done (fetches queued after it do
+ * not invalidate them).  SPNS_ENONFINITE if the run diverged (the scalars
+ * are still written). */
+int spns_batch_slot_scalars(spns_batch *b, int32_t slot, spns_scalars *out);
+/* Queues copies of a slot's state (compact layout, 6*nmodes doubles) and
+ * spectrum (spns_nbins doubles) into host buffers on the batch's stream
+ * (either may be NULL); they are complete when spns_batch_query reports
+ * done.  With pinned buffers (spns_host_alloc) the copies do not block the
+ * caller; the slot must not be stepped until then. */
+int spns_batch_fetch(spns_batch *b, int32_t slot, double *state, double *ek);
 const char *spns_batch_last_error(const spns_batch *b);
+/* Pinned host memory (hipHostMalloc) for asynchronous copies. */
+int spns_host_alloc(int64_t bytes, void **out);
+int spns_host_free(void *p);
 
 /* ---- FFT_package drop-in (src/FFTW_package.h:21-71) on device buffers ----
  * complex layout [y][x][Mz/2+1] (FFTW_MPI_TRANSPOSED, one rank),

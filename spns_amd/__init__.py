@@ -65,7 +65,8 @@ EXPORTS = [
     "spns_fft_create", "spns_fft_destroy", "spns_fft_r2c", "spns_fft_c2r", "spns_fft_normalize",
     "spns_fft_synchronize", "spns_fft_last_error",
     "spns_batch_create", "spns_batch_destroy", "spns_batch_attach", "spns_batch_detach", "spns_batch_step",
-    "spns_batch_query", "spns_batch_synchronize", "spns_batch_last_error",
+    "spns_batch_query", "spns_batch_synchronize", "spns_batch_slot_scalars", "spns_batch_fetch", "spns_batch_last_error",
+    "spns_host_alloc", "spns_host_free",
 ]
 
 
@@ -131,6 +132,10 @@ def load_library(path=LIB_PATH):
         "spns_batch_query": (I32, [P, ctypes.POINTER(I32)]),
         "spns_batch_synchronize": (I32, [P]),
         "spns_batch_last_error": (ctypes.c_char_p, [P]),
+        "spns_batch_slot_scalars": (I32, [P, I32, ctypes.POINTER(Scalars)]),
+        "spns_batch_fetch": (I32, [P, I32, D, D]),
+        "spns_host_alloc": (I32, [I64, ctypes.POINTER(P)]),
+        "spns_host_free": (I32, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -484,6 +489,20 @@ class Batch:
         self._chk(self.L.spns_batch_query(self.h, ctypes.byref(d)))
         return bool(d.value)
 
+    def slot_scalars(self, slot):
+        """Scalars of a slot after the last chunk (no device read; the batch
+        must be idle).  Raises SpnsError (SPNS_ENONFINITE) if it diverged."""
+        sc = Scalars()
+        self._chk(self.L.spns_batch_slot_scalars(self.h, slot, ctypes.byref(sc)))
+        return {k: getattr(sc, k) for k, _ in Scalars._fields_}
+
+    def fetch(self, slot, state=None, ek=None):
+        """Queues copies of a slot's state and spectrum into (pinned) host
+        arrays; complete once query() is True."""
+        sp = _dp(state.view(np.float64).reshape(-1)) if state is not None else None
+        ep = _dp(ek) if ek is not None else None
+        self._chk(self.L.spns_batch_fetch(self.h, slot, sp, ep))
+
     def synchronize(self):
         self._chk(self.L.spns_batch_synchronize(self.h))
 
@@ -491,6 +510,35 @@ class Batch:
         if getattr(self, "h", None):
             self.L.spns_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pinned:
+    """A numpy array in pinned host memory (spns_host_alloc), for
+    asynchronous device-to-host copies.  close() frees it; copy the data out
+    first."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.L = load_library()
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = ctypes.c_void_p()
+        rc = self.L.spns_host_alloc(nbytes, ctypes.byref(p))
+        if rc:
+            raise SpnsError(rc, self.L.spns_last_error(None).decode())
+        self.ptr = p
+        buf = (ctypes.c_char * nbytes).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            self.L.spns_host_free(self.ptr)
+            self.ptr = None
 
     def __del__(self):
         try:

@@ -851,15 +851,25 @@ struct spns_ctx {
 struct spns_batch {
     int device = 0, nslots = 0, chunk = 0;
     hipStream_t st = nullptr;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr;     // after the last queued work (chunk, uploads or fetches)
+    hipEvent_t ev_sc = nullptr;  // after the last copy of the slots' scalars (hsc)
     std::vector<spns_ctx*> slot;
     spns_ctx* rep = nullptr;  // a slot context giving the class's shapes
-    // per-slot argument structs, host copies and device arrays
-    std::vector<StridedArgs> h_invy, h_invx, h_fwdx, h_fwdy;
-    std::vector<ZArgs> h_z;
-    std::vector<EpiArgs> h_epi;
-    std::vector<FinArgs> h_fin;
-    std::vector<ClearArgs> h_clr;
+    // per-slot argument structs: pinned host copies (so their uploads are
+    // asynchronous on the batch's stream) and device arrays
+    struct SlotArgs {
+        StridedArgs invy, invx, fwdx, fwdy;
+        ZArgs z;
+        EpiArgs epi;
+        FinArgs fin;
+        ClearArgs clr;
+    };
+    SlotArgs* hs = nullptr;
+    // pinned copies of each slot's device scalars, refreshed after every
+    // chunk on the batch's stream: the loop reads a run's status without a
+    // synchronous device read (which would wait behind other streams' work
+    // sharing its hardware queue)
+    DevScalars* hsc = nullptr;
     StridedArgs *d_invy = nullptr, *d_invx = nullptr, *d_fwdx = nullptr, *d_fwdy = nullptr;
     ZArgs* d_z = nullptr;
     EpiArgs* d_epi = nullptr;
@@ -875,12 +885,14 @@ struct spns_batch {
         if (ch < 1) throw Error(SPNS_EINVAL, "chunk must be >= 1");
         device = dev; nslots = n; chunk = ch;
         slot.assign(n, nullptr);
-        h_invy.assign(n, StridedArgs{}); h_invx.assign(n, StridedArgs{}); h_fwdx.assign(n, StridedArgs{});
-        h_fwdy.assign(n, StridedArgs{}); h_z.assign(n, ZArgs{}); h_epi.assign(n, EpiArgs{});
-        h_fin.assign(n, FinArgs{}); h_clr.assign(n, ClearArgs{});
         HIPCHK(hipSetDevice(device));
+        HIPCHK(hipHostMalloc((void**)&hs, sizeof(SlotArgs) * n, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&hsc, sizeof(DevScalars) * n, hipHostMallocDefault));
+        std::memset((void*)hs, 0, sizeof(SlotArgs) * n);
+        std::memset((void*)hsc, 0, sizeof(DevScalars) * n);
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_sc, hipEventDisableTiming));
         auto al = [&](void** p, size_t bytes) {
             hipError_t e = hipMalloc(p, bytes);
             if (e != hipSuccess) { *p = nullptr; throw Error(SPNS_ENOMEM, "batch argument arrays: " + std::string(hipGetErrorString(e))); }
@@ -901,7 +913,10 @@ struct spns_batch {
         if (graph) hipGraphExecDestroy(graph);
         hipFree(d_invy); hipFree(d_invx); hipFree(d_fwdx); hipFree(d_fwdy); hipFree(d_z); hipFree(d_epi);
         hipFree(d_fin); hipFree(d_clr); hipFree(d_left);
+        if (hs) hipHostFree(hs);
+        if (hsc) hipHostFree(hsc);
         if (ev) hipEventDestroy(ev);
+        if (ev_sc) hipEventDestroy(ev_sc);
         if (st) hipStreamDestroy(st);
     }
 
@@ -926,23 +941,26 @@ struct spns_batch {
         if (r && !same_class(r, c)) throw Error(SPNS_EINVAL, "context does not match the batch's shape class");
         if (!batch_length(c->Mx) || !batch_length(c->My) || !batch_length(c->Mz))
             throw Error(SPNS_EUNSUPPORTED, "no batched kernels for this grid's FFT lengths");
-        HIPCHK(hipStreamSynchronize(st));        // no chunk in flight reads the arrays
+        HIPCHK(hipStreamSynchronize(st));        // no copy in flight reads this slot's pinned structs
         HIPCHK(hipStreamSynchronize(c->st));     // the context's own work is done
-        h_invy[s] = c->inv_y_args(false);
-        h_invx[s] = c->inv_x_args(false);
-        h_z[s] = c->zpass_args(nullptr);
-        h_fwdx[s] = c->fwd_x_args();
-        h_fwdy[s] = c->fwd_y_args(c->nfwd);
-        h_epi[s] = c->epi_args(EPI_STEP);
-        FinArgs f = c->fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE);
-        f.nblk_epi = c->grid_yepi > 0 ? c->grid_yepi : c->grid_epi;
-        h_fin[s] = f;
-        h_clr[s] = ClearArgs{c->mv, c->mv + 1};
-        auto up = [&](void* d, const void* h, size_t bytes) { HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice)); };
-        up(d_invy + s, &h_invy[s], sizeof(StridedArgs)); up(d_invx + s, &h_invx[s], sizeof(StridedArgs));
-        up(d_z + s, &h_z[s], sizeof(ZArgs)); up(d_fwdx + s, &h_fwdx[s], sizeof(StridedArgs));
-        up(d_fwdy + s, &h_fwdy[s], sizeof(StridedArgs)); up(d_epi + s, &h_epi[s], sizeof(EpiArgs));
-        up(d_fin + s, &h_fin[s], sizeof(FinArgs)); up(d_clr + s, &h_clr[s], sizeof(ClearArgs));
+        SlotArgs& a = hs[s];
+        a.invy = c->inv_y_args(false);
+        a.invx = c->inv_x_args(false);
+        a.z = c->zpass_args(nullptr);
+        a.fwdx = c->fwd_x_args();
+        a.fwdy = c->fwd_y_args(c->nfwd);
+        a.epi = c->epi_args(EPI_STEP);
+        a.fin = c->fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE);
+        a.fin.nblk_epi = c->grid_yepi > 0 ? c->grid_yepi : c->grid_epi;
+        a.clr = ClearArgs{c->mv, c->mv + 1};
+        auto up = [&](void* d, const void* h, size_t bytes) { HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st)); };
+        up(d_invy + s, &a.invy, sizeof(StridedArgs)); up(d_invx + s, &a.invx, sizeof(StridedArgs));
+        up(d_z + s, &a.z, sizeof(ZArgs)); up(d_fwdx + s, &a.fwdx, sizeof(StridedArgs));
+        up(d_fwdy + s, &a.fwdy, sizeof(StridedArgs)); up(d_epi + s, &a.epi, sizeof(EpiArgs));
+        up(d_fin + s, &a.fin, sizeof(FinArgs)); up(d_clr + s, &a.clr, sizeof(ClearArgs));
+        HIPCHK(hipMemcpyAsync(hsc + s, c->scal, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(ev_sc, st));
+        HIPCHK(hipEventRecord(ev, st));
         slot[s] = c;
         rep = r ? r : c;
     }
@@ -970,24 +988,24 @@ struct spns_batch {
         b = bl(d_clr);
         HIPCHK(launch_clear(nullptr, nullptr, st, &b));
         b = bl(d_invy);
-        HIPCHK(launch_strided(c->My, p, 1, p == PREC_FP64, +1, h_invy[slot_of(c)], c->nk1, st, &b));
+        HIPCHK(launch_strided(c->My, p, 1, p == PREC_FP64, +1, hs[slot_of(c)].invy, c->nk1, st, &b));
         b = bl(d_invx);
-        HIPCHK(launch_strided(c->Mx, p, p == PREC_FP64, p == PREC_FP64, +1, h_invx[slot_of(c)], c->My, st, &b));
+        HIPCHK(launch_strided(c->Mx, p, p == PREC_FP64, p == PREC_FP64, +1, hs[slot_of(c)].invx, c->My, st, &b));
         b = bl(d_z);
-        HIPCHK(launch_zpass(c->Mz, p, 0, h_z[slot_of(c)], c->grid_z, st, &b));
+        HIPCHK(launch_zpass(c->Mz, p, 0, hs[slot_of(c)].z, c->grid_z, st, &b));
         b = bl(d_fwdx);
-        HIPCHK(launch_strided(c->Mx, p, p == PREC_FP64, p == PREC_FP64, -1, h_fwdx[slot_of(c)], c->My, st, &b));
+        HIPCHK(launch_strided(c->Mx, p, p == PREC_FP64, p == PREC_FP64, -1, hs[slot_of(c)].fwdx, c->My, st, &b));
         if (c->grid_yepi > 0) {
             b = bl(d_fwdy, d_epi);
-            HIPCHK(launch_yepi(c->My, p, h_fwdy[slot_of(c)], c->nk1, h_epi[slot_of(c)], c->grid_yepi, st, &b));
+            HIPCHK(launch_yepi(c->My, p, hs[slot_of(c)].fwdy, c->nk1, hs[slot_of(c)].epi, c->grid_yepi, st, &b));
         } else {
             b = bl(d_fwdy);
-            HIPCHK(launch_strided(c->My, p, p == PREC_FP64, p == PREC_FP64, -1, h_fwdy[slot_of(c)], c->nk1, st, &b));
+            HIPCHK(launch_strided(c->My, p, p == PREC_FP64, p == PREC_FP64, -1, hs[slot_of(c)].fwdy, c->nk1, st, &b));
             b = bl(d_epi);
-            HIPCHK(launch_epilogue(p, h_epi[slot_of(c)], c->grid_epi, st, &b));
+            HIPCHK(launch_epilogue(p, hs[slot_of(c)].epi, c->grid_epi, st, &b));
         }
         b = bl(d_fin);
-        HIPCHK(launch_finalize(h_fin[slot_of(c)], st, &b));
+        HIPCHK(launch_finalize(hs[slot_of(c)].fin, st, &b));
     }
 
     int slot_of(const spns_ctx* c) const
@@ -1027,6 +1045,37 @@ struct spns_batch {
         }
         HIPCHK(launch_set_left(d_left, n, nslots, st));
         HIPCHK(hipGraphLaunch(graph, st));
+        for (int s = 0; s < nslots; s++)
+            if (slot[s] && n.n[s] > 0) HIPCHK(hipMemcpyAsync(hsc + s, slot[s]->scal, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(ev_sc, st));
+        HIPCHK(hipEventRecord(ev, st));
+    }
+
+    // the slots' scalar copies (hsc) have landed; fetches queued after them
+    // may still be in flight
+    bool scalars_ready()
+    {
+        const hipError_t e = hipEventQuery(ev_sc);
+        if (e == hipErrorNotReady) {
+            (void)hipGetLastError();
+            return false;
+        }
+        HIPCHK(e);
+        return true;
+    }
+
+    // the slot's state (compact layout) and spectrum copied into host
+    // buffers on the batch's stream (asynchronous when they are pinned)
+    void fetch(int s, double* state, double* ek)
+    {
+        if (s < 0 || s >= nslots || !slot[s]) throw Error(SPNS_EINVAL, "no context in this slot");
+        spns_ctx* c = slot[s];
+        if (state)
+            for (int q = 0; q < 3; q++)
+                HIPCHK(hipMemcpy2DAsync(state + 2 * (size_t)q * c->nmodes, sizeof(cx<double>) * c->nk3, c->uk[q],
+                                        sizeof(cx<double>) * c->nk3p, sizeof(cx<double>) * c->nk3, (size_t)c->nk2 * c->nk1,
+                                        hipMemcpyDeviceToHost, st));
+        if (ek) HIPCHK(hipMemcpyAsync(ek, c->ek, sizeof(double) * c->nbins, hipMemcpyDeviceToHost, st));
         HIPCHK(hipEventRecord(ev, st));
     }
 };
@@ -1377,11 +1426,9 @@ int spns_energy_cascade(spns_ctx* ctx, double* ekh, int32_t nbins, double* ek_to
     API_END(ctx)
 }
 
-int spns_get_scalars(spns_ctx* ctx, spns_scalars* out)
+// spns_scalars from the device scalars; SPNS_ENONFINITE if the run diverged
+static void to_scalars(const spns_ctx* ctx, const DevScalars& s, spns_scalars* out)
 {
-    API_BEGIN(ctx)
-    if (!out) throw Error(SPNS_EINVAL, "null output");
-    DevScalars s = ctx->read_scalars();
     out->time = s.time;
     out->dt = s.dt;
     out->nu = s.nu;
@@ -1394,7 +1441,15 @@ int spns_get_scalars(spns_ctx* ctx, spns_scalars* out)
     out->skewness = m2 > 0 ? m3 / std::pow(m2, 1.5) : 0.0;
     out->kappa = s.kappa;
     out->iter = s.iter;
-    if (!std::isfinite(s.maxvel) || !std::isfinite(s.ek)) throw Error(SPNS_ENONFINITE, "non-finite velocity field");
+    if (!std::isfinite(s.maxvel) || !std::isfinite(s.ek) || !std::isfinite(s.om) || !std::isfinite(s.dt))
+        throw Error(SPNS_ENONFINITE, "non-finite velocity field after step " + std::to_string(s.iter));
+}
+
+int spns_get_scalars(spns_ctx* ctx, spns_scalars* out)
+{
+    API_BEGIN(ctx)
+    if (!out) throw Error(SPNS_EINVAL, "null output");
+    to_scalars(ctx, ctx->read_scalars(), out);
     API_END(ctx)
 }
 
@@ -1827,6 +1882,42 @@ int spns_batch_synchronize(spns_batch* b)
     BATCH_END(b)
 }
 
+int spns_batch_slot_scalars(spns_batch* b, int32_t slot, spns_scalars* out)
+{
+    BATCH_BEGIN(b)
+    if (!out || slot < 0 || slot >= b->nslots || !b->slot[slot]) throw Error(SPNS_EINVAL, "no context in this slot");
+    if (!b->scalars_ready()) throw Error(SPNS_EINVAL, "the batch is still running");
+    to_scalars(b->slot[slot], b->hsc[slot], out);
+    BATCH_END(b)
+}
+
+int spns_batch_fetch(spns_batch* b, int32_t slot, double* state, double* ek)
+{
+    BATCH_BEGIN(b)
+    HIPCHK(hipSetDevice(b->device));
+    b->fetch(slot, state, ek);
+    BATCH_END(b)
+}
+
 const char* spns_batch_last_error(const spns_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int spns_host_alloc(int64_t bytes, void** out)
+{
+    if (!out || bytes <= 0) return SPNS_EINVAL;
+    *out = nullptr;
+    if (hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        g_create_error = "hipHostMalloc of " + std::to_string(bytes) + " bytes failed";
+        return SPNS_ENOMEM;
+    }
+    return SPNS_OK;
+}
+
+int spns_host_free(void* p)
+{
+    if (!p) return SPNS_EINVAL;
+    return hipHostFree(p) == hipSuccess ? SPNS_OK : SPNS_EHIP;
+}
 
 }  // extern "C"

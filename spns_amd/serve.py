@@ -33,7 +33,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import PREC_FP32_SPLIT, Batch, Solver, SpnsError, device_bytes, device_memory
+from . import PREC_FP32_SPLIT, Batch, Pinned, Solver, SpnsError, device_bytes, device_memory
 
 GRIDS = [(32, 32, 32), (64, 64, 64), (96, 64, 64), (128, 128, 128), (192, 128, 128)]
 
@@ -217,15 +217,21 @@ class Server:
     def _pool_key(self, args, v):
         return (args["Nx"], args["Ny"], args["Nz"], args["precision"], args["C_Smag"] > 0.0, v)
 
+    @staticmethod
+    def _close_entry(e):
+        e["solver"].close()
+        for p in e.get("pin") or ():
+            p.close()
+
     def _evict(self, need):
         """Frees kept contexts (oldest first) until need bytes fit the budget."""
         while self.used + need > self.budget and self._kept:
             key = next(iter(self._kept))
-            s, nbytes = self._kept[key].pop(0)
+            e = self._kept[key].pop(0)
             if not self._kept[key]:
                 del self._kept[key]
-            s.close()
-            self.used -= nbytes
+            self._close_entry(e)
+            self.used -= e["bytes"]
 
     def _start(self, req, now):
         args = req.solver_args(self.precision)
@@ -234,15 +240,17 @@ class Server:
         key = self._pool_key(args, v)
         nbytes = device_bytes(zpass_variant=v, **args)
         kept = self._kept.get(key)
+        pin = None
         if kept:
-            s, nbytes = kept.pop(0)
+            e = kept.pop(0)
             if not kept:
                 del self._kept[key]
+            s, nbytes, pin = e["solver"], e["bytes"], e.get("pin")
             self.reused += 1
             try:
                 s.reconfigure(**args)
             except SpnsError:
-                s.close()
+                self._close_entry(e)
                 self.used -= nbytes
                 raise
         else:
@@ -252,21 +260,30 @@ class Server:
             s.init_random(seed=req.seed, k0=req.k0)
             s.finish_init(False)
         except SpnsError:
-            s.close()
+            self._close_entry({"solver": s, "pin": pin})
             self.used -= nbytes
             raise
         req.admitted = now
-        return {"req": req, "solver": s, "done": 0, "waited": 0, "bytes": nbytes, "key": key}
+        return {"req": req, "solver": s, "done": 0, "waited": 0, "bytes": nbytes, "key": key, "pin": pin}
 
     def _release(self, a, keep):
         s = a.get("solver")
         if s is None:
             return
+        e = {"solver": s, "bytes": a["bytes"], "pin": a.get("pin")}
         if keep and self.keep_contexts:
-            self._kept.setdefault(a["key"], []).append((s, a["bytes"]))
+            self._kept.setdefault(a["key"], []).append(e)
         else:
-            s.close()
+            self._close_entry(e)
             self.used -= a["bytes"]
+
+    def _finish_from(self, a, sc, state, ek, now):
+        """Result of a batched run from its fetched (pinned) copies."""
+        req = a["req"]
+        req.result = {"ek": np.array(ek), "energy": sc["ek"], "epsilon": sc["nu"] * sc["om"],
+                      "skewness": sc["skewness"], "time": sc["time"], "steps": sc["iter"], "state": np.array(state)}
+        req.finished = now
+        self._release(a, keep=True)
 
     def _finish(self, a, now):
         s, req = a["solver"], a["req"]
@@ -289,9 +306,9 @@ class Server:
 
     def _close_kept(self):
         for lst in self._kept.values():
-            for s, nbytes in lst:
-                s.close()
-                self.used -= nbytes
+            for e in lst:
+                self._close_entry(e)
+                self.used -= e["bytes"]
         self._kept = {}
 
     @staticmethod
@@ -331,10 +348,12 @@ class Server:
             nonlocal n_runs
             g["batch"].detach(slot)
             del g["slots"][slot]
+            now = time.perf_counter() - t0
             if failed_err is not None:
-                self._fail(a, failed_err, time.perf_counter() - t0)
+                self._fail(a, failed_err, now)
             else:
-                self._finish(a, time.perf_counter() - t0)
+                st, ek = a["pin"]
+                self._finish_from(a, a["final"], st.array, ek.array, now)
                 if not keep_state:
                     a["req"].result.pop("state", None)
             n_runs -= 1
@@ -384,14 +403,24 @@ class Server:
                             g["busy"] = False
                         tc = clock()
                         for slot, a in list(g["slots"].items()):
+                            if "final" in a:  # its fetched results have landed
+                                retire(g, slot, a)
+                                progressed = True
+                                continue
                             try:
-                                a["solver"].query()  # raises SPNS_ENONFINITE if the run diverged
+                                sc = g["batch"].slot_scalars(slot)  # SPNS_ENONFINITE if it diverged
                             except SpnsError as e:
                                 retire(g, slot, a, e)
                                 progressed = True
                                 continue
                             if a["done"] >= a["req"].steps:
-                                retire(g, slot, a)
+                                # queue the copies of its results on the batch's stream;
+                                # it retires when they have landed (after the next chunk)
+                                s = a["solver"]
+                                if a.get("pin") is None:
+                                    a["pin"] = (Pinned((3, s.nmodes), np.complex128), Pinned((s.nbins,)))
+                                g["batch"].fetch(slot, a["pin"][0].array, a["pin"][1].array)
+                                a["final"] = sc
                                 progressed = True
                         host["retire"] += clock() - tc
                         tc = clock()
@@ -405,7 +434,8 @@ class Server:
                         host["attach"] += clock() - tc
                         plan = [0] * self.nslots
                         for slot, a in g["slots"].items():
-                            plan[slot] = min(self.chunk, a["req"].steps - a["done"])
+                            if "final" not in a:
+                                plan[slot] = min(self.chunk, a["req"].steps - a["done"])
                         if any(plan):
                             tc = clock()
                             g["batch"].step(plan)
@@ -415,6 +445,8 @@ class Server:
                                 a["done"] += plan[slot]
                                 steps_done += plan[slot]
                             progressed = True
+                        elif any("final" in a for a in g["slots"].values()):
+                            g["busy"] = True  # only fetches in flight
                 if not progressed:
                     tc = clock()
                     if not n_runs and pending:

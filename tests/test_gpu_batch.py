@@ -128,3 +128,33 @@ def test_batch_isolates_a_diverging_slot():
         ref.close()
     for x in [b, bad] + good:
         x.close()
+
+
+def test_batch_status_and_fetch_without_device_reads():
+    """spns_batch_slot_scalars (the copy the batch makes after each chunk) and
+    spns_batch_fetch into pinned host arrays give exactly what the context's
+    own synchronous reads give."""
+    import spns_amd
+    b = spns_amd.Batch(3, 8)
+    runs = [_start(20 + i, Nx=64, precision=1, nu=1e-3 * (i + 1), C_At=0.5) for i in range(2)]
+    b.attach(0, runs[0])
+    b.attach(2, runs[1])
+    b.step([8, 0, 5])
+    b.synchronize()
+    st = spns_amd.Pinned((3, runs[1].nmodes), np.complex128)
+    ek = spns_amd.Pinned((runs[1].nbins,))
+    b.fetch(2, st.array, ek.array)
+    b.synchronize()
+    for slot, s in ((0, runs[0]), (2, runs[1])):
+        fast, ref = b.slot_scalars(slot), s.scalars()
+        for k in ("time", "dt", "ek", "om", "skewness", "maxvel", "iter"):
+            assert fast[k] == ref[k], (slot, k)
+    assert np.array_equal(st.array, runs[1].get_state())
+    assert np.array_equal(ek.array, runs[1].spectrum())
+    st.close()
+    ek.close()
+    with pytest.raises(spns_amd.SpnsError):
+        b.slot_scalars(1)  # empty slot
+    b.close()
+    for s in runs:
+        s.close()

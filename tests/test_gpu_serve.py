@@ -21,6 +21,7 @@ def test_mix_equals_alone(batched):
     srv = serve.Server(hbm_budget=40e9, chunk=7, batched=batched)
     stats, done = srv.run(reqs, keep_state=True)
     assert stats["requests"] == 8
+    assert [r.error for r in done if r.error] == []
     assert stats["contexts_reused"] >= 1  # (all four when the first runs finish before the later ones arrive)
     for r in done:
         alone = serve.run_alone(r)
