@@ -171,7 +171,7 @@ def cpu_baseline(n_cpu, threads, steps, les=False):
     return ts[len(ts) // 2], ts, lib().orc_num_threads()
 
 
-def run_ensemble(n, seed, batched=True, rate=20.0):
+def run_ensemble(n, seed, batched=True, rate=20.0, chunk=20):
     """The serving mix (Poisson arrivals at `rate` req/s, mixed grids,
     DNS/LES, viscosities and rotation rates, 100-2000 steps each,
     FP32-split), first n requests: throughput and completion-latency
@@ -179,7 +179,7 @@ def run_ensemble(n, seed, batched=True, rate=20.0):
     class (spns_batch); else one graph per run on its own stream."""
     from spns_amd import serve
     reqs = serve.draw_requests(n, rate=rate, seed=seed, steps=(100, 2000))
-    srv = serve.Server(tuner=None if batched else serve.Tuner(), batched=batched)
+    srv = serve.Server(tuner=None if batched else serve.Tuner(), batched=batched, chunk=chunk)
     stats, done = srv.run(reqs)
     if batched:
         stats["policy"] = {"batched": True, "nslots": srv.nslots, "chunk": srv.chunk}
@@ -247,6 +247,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--ensemble-requests", type=int, default=32,
                     help="requests of the serving mix to run (0: skip)")
+    ap.add_argument("--ensemble-rate", type=float, default=20.0, help="Poisson arrival rate of the serving mix (req/s)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving mix (--ensemble-requests at --ensemble-rate) and print its JSON line")
+    ap.add_argument("--chunk", type=int, default=20, help="steps per batched chunk in the serving loop")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32")
     ap.add_argument("--les", action="store_true",
@@ -262,6 +266,11 @@ def main():
         return
     if a.copy_peak:
         print(json.dumps(copy_peak()))
+        return
+    if a.serving_only:
+        out = run_ensemble(a.ensemble_requests, a.seed, rate=a.ensemble_rate, chunk=a.chunk)
+        out["metric"] = "serving mix: p50/p99 completion latency at a steady Poisson rate"
+        print(json.dumps(out))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: several GPUs would shard the step as a slab decomposition "
@@ -347,7 +356,7 @@ def main():
     ens = None
     if a.ensemble_requests > 0:
         print("bench: ensemble mix, %d requests" % a.ensemble_requests, file=sys.stderr, flush=True)
-        ens = run_ensemble(a.ensemble_requests, a.seed)
+        ens = run_ensemble(a.ensemble_requests, a.seed, rate=a.ensemble_rate, chunk=a.chunk)
         print("bench: ensemble %.1f s wall, p50 %.2f s, p99 %.2f s" % (ens["wall_s"], ens["latency_p50_s"],
               ens["latency_p99_s"]), file=sys.stderr, flush=True)
 

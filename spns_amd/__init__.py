@@ -57,7 +57,7 @@ EXPORTS = [
     "spns_set_state", "spns_get_state", "spns_get_state_planes", "spns_get_rhist", "spns_init_from_physical", "spns_init_k41",
     "spns_init_random", "spns_finish_init", "spns_step", "spns_recalculate_timestep", "spns_set_nu",
     "spns_forcing_reynolds_lambda", "spns_recalculate_reynolds_lambda", "spns_forcing_energy_cascade",
-    "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_get_scalars",
+    "spns_complex_conjugate_correction", "spns_energy_cascade", "spns_pass_kernels", "spns_get_scalars",
     "spns_velocity_physical", "spns_velocity_physical_step_start", "spns_check_phys_four_energy",
     "spns_recalculate_velocity_antitransform", "spns_synchronize", "spns_query", "spns_device_memory", "spns_copy_bandwidth", "spns_last_step_ms", "spns_invariant_spectra",
     "spns_invariant_spectra_noalloc",
@@ -104,6 +104,7 @@ def load_library(path=LIB_PATH):
         "spns_complex_conjugate_correction": (I32, [P]),
         "spns_energy_cascade": (I32, [P, D, I32, D]),
         "spns_get_scalars": (I32, [P, ctypes.POINTER(Scalars)]),
+        "spns_pass_kernels": (I32, [P, ctypes.POINTER(I32), ctypes.POINTER(I32)]),
         "spns_velocity_physical": (I32, [P, D]),
         "spns_velocity_physical_step_start": (I32, [P, D]),
         "spns_recalculate_velocity_antitransform": (I32, [P]),
@@ -388,6 +389,13 @@ class Solver:
 
     def enstrophy(self):
         return self.scalars()["om"]
+
+    def pass_kernels(self):
+        """(z-pass kernel: 2 staged, 1 block per pencil, 0 wave per pencil;
+        True if the forward y pass is fused with the epilogue)."""
+        z, f = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.L.spns_pass_kernels(self.h, ctypes.byref(z), ctypes.byref(f)))
+        return z.value, bool(f.value)
 
     def scalars(self):
         s = Scalars()

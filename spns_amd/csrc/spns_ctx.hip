@@ -1427,6 +1427,8 @@ int spns_energy_cascade(spns_ctx* ctx, double* ekh, int32_t nbins, double* ek_to
 }
 
 // spns_scalars from the device scalars; SPNS_ENONFINITE if the run diverged
+static int zpass_kernel_of(int L, int prec, const ZArgs& z) { return zpass_kernel(L, prec, z); }
+
 static void to_scalars(const spns_ctx* ctx, const DevScalars& s, spns_scalars* out)
 {
     out->time = s.time;
@@ -1443,6 +1445,18 @@ static void to_scalars(const spns_ctx* ctx, const DevScalars& s, spns_scalars* o
     out->iter = s.iter;
     if (!std::isfinite(s.maxvel) || !std::isfinite(s.ek) || !std::isfinite(s.om) || !std::isfinite(s.dt))
         throw Error(SPNS_ENONFINITE, "non-finite velocity field after step " + std::to_string(s.iter));
+}
+
+int spns_pass_kernels(spns_ctx* ctx, int32_t* zpass_kernel, int32_t* fused_epilogue)
+{
+    API_BEGIN(ctx)
+    if (zpass_kernel) {
+        ZArgs z = ctx->zpass_args(nullptr);
+        z.kernel = ctx->zkernel;
+        *zpass_kernel = zpass_kernel_of(ctx->Mz, ctx->prec, z);
+    }
+    if (fused_epilogue) *fused_epilogue = ctx->grid_yepi > 0 ? 1 : 0;
+    API_END(ctx)
 }
 
 int spns_get_scalars(spns_ctx* ctx, spns_scalars* out)

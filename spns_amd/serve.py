@@ -3,29 +3,34 @@
 Requests (independent runs of mixed grids, viscosities, Smagorinsky
 constants and rotation rates) arrive over time.  The loop
   * admits a request when it has arrived and its device footprint
-    (spns_device_bytes) fits the HBM budget left by the runs in flight;
+    (spns_device_bytes) fits the HBM budget left by the runs in flight (the
+    budget from spns_device_memory);
   * keeps every admitted run resident: state, history and work buffers stay
     in HBM from admission to completion;
-  * advances admitted runs in chunks of hipGraph-captured steps on each
-    run's own stream; the loop polls each run (spns_query) and gives the
-    runs with the least remaining work their next chunk as soon as their
-    previous one has finished (a per-step cost budget of in-flight runs, at
-    least six), so the runs' kernels overlap on the device and short runs
-    are not held behind long ones;
-  * retires runs that reached their step count (collecting E(k), energy,
-    dissipation and skewness), frees their memory and admits queued runs
-    into the freed space while the others continue;
-  * retires a run that diverged (spns_query reports SPNS_ENONFINITE after
-    the chunk in which its field went NaN/Inf) or failed to start, with an
-    error result, and keeps the others going: one bad request never takes
-    the loop down (the reference's crash() aborts every rank instead,
-    src/util.cpp:93-116).
+  * batched (the default): seats each run in a slot of its shape class's
+    Batch (grid, precision, DNS or LES) and advances every batch in chunks of
+    hipGraph-captured steps, each pass one kernel launch over all the
+    batch's runs; batches of different classes run concurrently on their own
+    streams, and runs join and leave batches between chunks;
+  * reads each run's status after every chunk from pinned copies the batch
+    makes on its own stream, and a finished run's results the same way, so
+    the loop never waits on a device read behind other streams' work;
+  * retires runs that reached their step count (E(k), energy, dissipation,
+    skewness, state) and keeps their contexts for later runs of the same
+    class (spns_reconfigure) instead of freeing device memory, which would
+    wait for all work on the device;
+  * retires a run that diverged (NaN/Inf: SPNS_ENONFINITE) or failed to
+    start, with an error result, and keeps the others going: one bad request
+    never takes the loop down (the reference's crash() aborts every rank
+    instead, src/util.cpp:93-116).
+Per-run mode (batched=False): one graph per run on its own stream, polled
+with spns_query, shortest remaining work first with aging, and a timing
+autotuner (Tuner) for the z-pass kernel of each shape class.
 Latency = completion time - arrival time; the loop reports p50/p99 beside
-throughput.  The z-pass kernel variant of each shape class is picked by a
-small timing autotuner (Tuner) before the loop starts, on an idle device.
+throughput.
 
-Runs never share state: a run's result is the same alone and in the mix
-(tests/test_gpu_serve.py checks it bit for bit).
+Runs never share state: a run's result is the same alone and in the mix, bit
+for bit, batched or not (tests/test_gpu_serve.py, tests/test_gpu_batch.py).
 """
 import math
 import time
@@ -190,7 +195,7 @@ class Server:
 
     def __init__(self, hbm_budget=None, precision=PREC_FP32_SPLIT, chunk=20, max_active=128, tuner=None,
                  inflight_work=2e7, min_inflight=6, max_wait=16, device=0, batched=True, nslots=16,
-                 keep_contexts=True):
+                 keep_contexts=True, admit_per_pass=1):
         if hbm_budget is None:
             hbm_budget = device_memory(device)[0] * 0.8
         self.budget = hbm_budget
@@ -205,6 +210,7 @@ class Server:
         self.batched = batched
         self.nslots = nslots
         self.keep_contexts = keep_contexts
+        self.admit_per_pass = max(1, int(admit_per_pass))
         self.used = 0.0
         self.reused = 0
         self._kept = {}
@@ -363,9 +369,14 @@ class Server:
             while pending or n_runs:
                 now = time.perf_counter() - t0
                 progressed = False
-                # admission by HBM footprint, in arrival order; the run is
-                # initialised now and waits for a slot of its class's batch
-                while pending and pending[0].arrival <= now and n_runs < self.max_active:
+                # admission by HBM footprint, in arrival order, at most
+                # admit_per_pass runs per pass (an initialisation takes tens of
+                # ms; the running batches are relaunched in between); the run
+                # is initialised now and waits for a slot of its class's batch
+                admitted = 0
+                while (pending and pending[0].arrival <= now and n_runs < self.max_active
+                       and admitted < self.admit_per_pass):
+                    admitted += 1
                     r = pending[0]
                     args = r.solver_args(self.precision)
                     need = 0.0 if self._kept.get(self._pool_key(args, 0)) else device_bytes(**args)

@@ -201,6 +201,7 @@ def test_fused_epilogue_matches_split(monkeypatch, kw, precision):
         else:
             monkeypatch.delenv("SPNS_EPI", raising=False)
         g = _solver(precision=precision, **kw)
+        assert g.pass_kernels()[1] == (mode == "fused")  # the fused kernel really runs (or really does not)
         g.init_random(seed=9, k0=4.0)
         g.finish_init(False)
         g.step(6)
