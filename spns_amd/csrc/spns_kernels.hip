@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "spns_device.hpp"
 #include "spns_internal.hpp"
@@ -129,39 +130,57 @@ __host__ __device__ constexpr int sstage_nt()
     return m > 1024 ? 1024 : (m < 64 ? 64 : m);
 }
 
-template <typename TIN, typename T, int L, int TB, int DIR>
+// NBM = 2: the transform holds 2 TB columns (the combined dual job below).
+template <typename TIN, typename T, int L, int TB, int DIR, int NBM = 1>
 __host__ __device__ constexpr size_t sstage_lds_bytes()
 {
-    return sizeof(cx<T>) * ((size_t)L * TB + fft_twiddle_span<L, sizeof(T) == 8 ? 1 : 2>()) +
+    return sizeof(cx<T>) * ((size_t)L * TB * NBM + fft_twiddle_span<L, sizeof(T) == 8 ? 1 : 2>()) +
            sizeof(cx<TIN>) * (size_t)sstage_rows<L, DIR>() * TB;
 }
 
-template <typename TIN, typename T, int L, int TB, int DIR>
+template <typename TIN, typename T, int L, int TB, int DIR, int NBM = 1>
 __host__ __device__ constexpr bool sstage_fits()
 {
-    return sstage_lds_bytes<TIN, T, L, TB, DIR>() <= 160 * 1024 - 512 && (TB * sizeof(cx<TIN>)) % 16 == 0;
+    return sstage_lds_bytes<TIN, T, L, TB, DIR, NBM>() <= 160 * 1024 - 512 && (TB * sizeof(cx<TIN>)) % 16 == 0;
 }
 
 // STDP: inverse pass at the standard 3/2 padding (nin = 2 (L/3) + 1 stored
 // rows) with L/R0 = 64 or 128 and one stage-0 butterfly per thread; the zone
 // of each stage-0 input (stored low rows, stored high rows, zero padding) is
 // then fixed at compile time.
-template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
+// JSEL picks the jobs a launch takes: 0 all of them; 1 the dual jobs, each
+// item as one transform of 2 TB columns (columns b < TB plain, b >= TB the
+// i k transform of column b - TB, both formed from one staged tile), so the
+// next item's copy is issued after the one shared stage 0 and stays in
+// flight across both transforms; 2 the single jobs.
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false, int JSEL = 0>
 __device__ __forceinline__ void sstage_body(const StridedArgs& a)
 {
+    constexpr int NB = JSEL == 1 ? 2 * TB : TB;  // columns of the transform (and LDS row pitch of fbuf)
+    static_assert((TB & (TB - 1)) == 0, "column tile must be a power of two");
     constexpr int SROWS = sstage_rows<L, DIR>();
     constexpr int SPR = TB * (int)sizeof(cx<TIN>) / 16;  // 16-byte copy slots per staged row
     constexpr int EPS = 16 / (int)sizeof(cx<TIN>);        // input elements per slot
     constexpr int NW = NT / 64;
     constexpr int TWM = sizeof(T) == 8 ? 1 : 2;
     constexpr int TWN = fft_twiddle_span<L, TWM>();  // the table entries the transform reads
-    __shared__ __align__(16) cx<T> fbuf[L * TB];
+    __shared__ __align__(16) cx<T> fbuf[L * NB];
     __shared__ __align__(16) cx<TIN> stg[SROWS * TB];
     __shared__ __align__(16) cx<T> twl[TWN];
     const int t = threadIdx.x;
     const int hin = a.nin / 2, hout = a.nout / 2;
     const int ntiles = (a.B + TB - 1) / TB;
-    const int64_t per_a = (int64_t)a.njobs * ntiles;
+    // the launch's jobs as 4-bit job indices (at most 9 jobs)
+    uint64_t jsel = 0;
+    int nsel = 0;
+    for (int j = 0; j < a.njobs; j++) {
+        const bool d = a.job[j].out2 != nullptr;
+        if (JSEL == 0 || (JSEL == 1) == d) {
+            jsel |= (uint64_t)j << (4 * nsel);
+            nsel++;
+        }
+    }
+    const int64_t per_a = (int64_t)nsel * ntiles;
     const int64_t nitems = (int64_t)a.A * per_a;
     const int G = gridDim.x;  // a multiple of 8
     const int64_t lin = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
@@ -175,7 +194,8 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
     auto issue = [&](int64_t it) {
         const int64_t ai = it / per_a;
         const int r = (int)(it - ai * per_a);
-        const int jb = r / ntiles, b0 = (r - jb * ntiles) * TB;
+        const int js = r / ntiles, b0 = (r - js * ntiles) * TB;
+        const int jb = (int)((jsel >> (4 * js)) & 15);
         const cx<TIN>* in = (const cx<TIN>*)a.job[jb].in + ai * a.in_a_stride + b0;
         for (int m = wave; m * 64 < nslot; m += NW) {
             const int s = m * 64 + lane;
@@ -191,8 +211,9 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
     for (; it < nitems; it += G) {
         const int64_t ai = it / per_a;
         const int r = (int)(it - ai * per_a);
-        const int jb = r / ntiles, b0 = (r - jb * ntiles) * TB;
-        const bool dual = a.job[jb].out2 != nullptr;
+        const int js = r / ntiles, b0 = (r - js * ntiles) * TB;
+        const int jb = (int)((jsel >> (4 * js)) & 15);
+        const bool dual = JSEL == 1 || (JSEL == 0 && a.job[jb].out2 != nullptr);
         int deriv = dual ? 0 : a.job[jb].deriv;
         const int64_t nxt = it + G;
         auto load = [&](int b, int p) -> cx<T> {
@@ -201,8 +222,8 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
             else if (p <= hin) { i = p; k = p; }
             else if (p >= L - hin) { i = a.nin - (L - p); k = p - L; }
             else return cx<T>{(T)0, (T)0};
-            const cx<TIN> v = stg[i * TB + b];
-            if (deriv) {
+            const cx<TIN> v = stg[i * TB + (b & (TB - 1))];
+            if (JSEL == 1 ? b >= TB : deriv) {
                 // i k u, formed in the input precision (src/HIT_LES.cpp:37)
                 const TIN kk = (TIN)k;
                 return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
@@ -226,8 +247,8 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
                     if (!(lo || hi)) return cx<T>{(T)0, (T)0};
                 }
                 const int i = lo ? p : p - (L - NIN);
-                const cx<TIN> v = stg[i * TB + b];
-                if (deriv) {
+                const cx<TIN> v = stg[i * TB + (b & (TB - 1))];
+                if (JSEL == 1 ? b >= TB : deriv) {
                     // i k u, formed in the input precision (src/HIT_LES.cpp:37)
                     const TIN kk = (TIN)(lo ? p : p - L);
                     return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
@@ -235,35 +256,41 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
                 return cx<T>{(T)v.x, (T)v.y};
             }
         };
-        auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * TB + b] = v; };
+        auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * NB + b] = v; };
         auto prefetch = [&]() {
             if (nxt < nitems) issue(nxt);
         };
-        auto write_tile = [&](void* dst) {
+        // output tile from fbuf columns [0, TB) (and [TB, 2 TB) into dst2)
+        auto write_tile = [&](void* dst, void* dst2) {
             cx<TOUT>* out = (cx<TOUT>*)dst + ai * a.out_a_stride + b0;
-            for (int idx = t; idx < a.nout * TB; idx += NT) {
-                const int i = idx / TB, b = idx - i * TB;
+            cx<TOUT>* out2 = (JSEL == 1) ? (cx<TOUT>*)dst2 + ai * a.out_a_stride + b0 : out;
+            for (int idx = t; idx < a.nout * NB; idx += NT) {
+                const int i = idx / NB, bq = idx - i * NB, b = bq & (TB - 1);
                 const int p = (a.nout == L || i <= hout) ? i : L - (a.nout - i);
                 if (b0 + b < a.B) {
-                    const cx<T> v = fbuf[p * TB + b];
-                    out[(int64_t)i * a.out_row_stride + b] = cx<TOUT>{(TOUT)v.x, (TOUT)v.y};
+                    const cx<T> v = fbuf[p * NB + bq];
+                    (bq >= TB ? out2 : out)[(int64_t)i * a.out_row_stride + b] = cx<TOUT>{(TOUT)v.x, (TOUT)v.y};
                 }
             }
         };
         auto fft = [&](auto lead, auto& hook) {
             constexpr bool LEAD = decltype(lead)::value;
             if constexpr (STDP)
-                FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<LEAD>(fbuf, twl, load_std, store_lds, hook, t);
+                FftS<T, L, NB, NT, NB, DIR, false, TWM>::template run<LEAD>(fbuf, twl, load_std, store_lds, hook, t);
             else
-                FftS<T, L, TB, NT, TB, DIR, false, TWM>::template run<LEAD>(fbuf, twl, load, store_lds, hook, t);
+                FftS<T, L, NB, NT, NB, DIR, false, TWM>::template run<LEAD>(fbuf, twl, load, store_lds, hook, t);
         };
-        if (dual) {
+        if constexpr (JSEL == 1) {
+            // both transforms of the dual job at once; the copy of the next
+            // item's tile is in flight from the end of stage 0 on
+            fft(std::false_type{}, prefetch);
+        } else if (dual) {
             // plain transform first, from the staged tile; the tile stays
             // staged (no prefetch yet) for the i k transform below
             auto none = [] {};
             fft(std::false_type{}, none);
             lds_barrier();
-            write_tile(a.job[jb].out);
+            write_tile(a.job[jb].out, nullptr);
             deriv = 1;
             fft(std::true_type{}, prefetch);
         } else {
@@ -271,22 +298,23 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
         }
         lds_barrier();
         dma_wait();  // the next item's tile (this wave's copies) has landed
-        write_tile(dual ? a.job[jb].out2 : a.job[jb].out);
+        if constexpr (JSEL == 1) write_tile(a.job[jb].out, a.job[jb].out2);
+        else write_tile(dual ? a.job[jb].out2 : a.job[jb].out, nullptr);
         lds_barrier();  // every wave's copies are in; fbuf is free again
     }
 }
 
-template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false, int JSEL = 0>
 __global__ __launch_bounds__(NT) void k_sstage(StridedArgs a)
 {
-    sstage_body<TIN, TOUT, T, L, TB, NT, DIR, STDP>(a);
+    sstage_body<TIN, TOUT, T, L, TB, NT, DIR, STDP, JSEL>(a);
 }
 
-template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false>
+template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DIR, bool STDP = false, int JSEL = 0>
 __global__ __launch_bounds__(NT) void k_sstage_b(const StridedArgs* __restrict__ arr, const int* __restrict__ left)
 {
     if (left[blockIdx.y] <= 0) return;
-    sstage_body<TIN, TOUT, T, L, TB, NT, DIR, STDP>(arr[blockIdx.y]);
+    sstage_body<TIN, TOUT, T, L, TB, NT, DIR, STDP, JSEL>(arr[blockIdx.y]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1939,6 +1967,78 @@ constexpr int zpass_nt()
     return L >= 768 ? 512 : (L >= 288 ? 256 : (L >= 144 ? 128 : 64));
 }
 
+// One launch of the staged kernel over the jobs JSEL selects (njsel of them).
+template <typename TIN, typename TOUT, typename T, int L, int TB, int DIR, int JSEL>
+static hipError_t launch_sstage(const StridedArgs& b, int A, int njsel, hipStream_t st, const BatchLaunch* bl)
+{
+    constexpr int NB = JSEL == 1 ? 2 * TB : TB;
+    constexpr int NT = sstage_nt<L, NB>();
+    constexpr int LR0 = L / Plan<L>::r(0);
+    constexpr bool STDP_OK = DIR > 0 && (LR0 == 64 || LR0 == 128) && NB * LR0 <= NT;
+    const bool stdp = STDP_OK && b.nin == 2 * (L / 3) + 1;
+    static const int occ = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sstage<TIN, TOUT, T, L, TB, NT, DIR, false, JSEL>, NT, 0) !=
+            hipSuccess)
+            n = 1;
+        return n > 0 ? n : 1;
+    }();
+    int dev = 0, ncu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const int64_t items = (int64_t)A * njsel * ((b.B + TB - 1) / TB);
+    int64_t g = (int64_t)ncu * occ;
+    if (items < g) g = items;
+    g = (g + 7) / 8 * 8;
+    if (bl) {
+        // batched: each slot's struct (with A set) is in bl->args
+        if constexpr (batch_length(L)) {
+            const dim3 grid((unsigned)g, (unsigned)bl->nslots);
+            const StridedArgs* arr = (const StridedArgs*)bl->args;
+            if constexpr (STDP_OK) {
+                if (stdp) {
+                    hipLaunchKernelGGL((k_sstage_b<TIN, TOUT, T, L, TB, NT, DIR, true, JSEL>), grid, dim3(NT), 0, st, arr, bl->left);
+                    return hipGetLastError();
+                }
+            }
+            hipLaunchKernelGGL((k_sstage_b<TIN, TOUT, T, L, TB, NT, DIR, false, JSEL>), grid, dim3(NT), 0, st, arr, bl->left);
+            return hipGetLastError();
+        }
+        return hipErrorNotSupported;
+    }
+    if constexpr (STDP_OK) {
+        if (stdp) {
+            hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR, true, JSEL>), dim3((unsigned)g), dim3(NT), 0, st, b);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR, false, JSEL>), dim3((unsigned)g), dim3(NT), 0, st, b);
+    return hipGetLastError();
+}
+
+// The combined dual-job kernel (JSEL 1) for the FP64 inverse passes at
+// L >= 512, unless SPNS_DUALC=0; the single jobs of the same pass then take
+// a second launch.  At 512^3 it took FP64 inv_x from 8.00 to 7.59 ms and
+// inv_y from 3.81 to 3.60 ms.  In FP32 it took inv_x from 4.01 to 3.76 ms,
+// but the z pass that follows ran 0.3 ms longer at the same cycle count
+// (rocprofv3 GRBM_GUI_ACTIVE: a lower engine clock after the faster
+// memory-bound kernel), so the FP32 step gained nothing and keeps JSEL 0.
+template <typename TIN, typename T, int L, int TB, int DIR>
+constexpr bool dualc_applies()
+{
+    return DIR > 0 && sizeof(T) == 8 && L >= 512 && sstage_fits<TIN, T, L, TB, DIR, 2>();
+}
+
+static bool dualc_enabled()
+{
+    static const bool on = [] {
+        const char* s = getenv("SPNS_DUALC");
+        return !(s && s[0] == '0');
+    }();
+    return on;
+}
+
 template <typename TIN, typename TOUT, typename T, int L, int DIR>
 static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st, const BatchLaunch* bl)
 {
@@ -1947,47 +2047,16 @@ static hipError_t launch_strided_L(const StridedArgs& a, int A, hipStream_t st, 
     b.A = A;
     if constexpr (sstage_fits<TIN, T, L, TB, DIR>()) {
         if (a.padded_ok && a.nin <= sstage_rows<L, DIR>()) {
-            constexpr int NT = sstage_nt<L, TB>();
-            constexpr int LR0 = L / Plan<L>::r(0);
-            constexpr bool STDP_OK = DIR > 0 && (LR0 == 64 || LR0 == 128) && TB * LR0 <= NT;
-            const bool stdp = STDP_OK && a.nin == 2 * (L / 3) + 1;
-            static const int occ = [] {
-                int n = 0;
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sstage<TIN, TOUT, T, L, TB, NT, DIR>, NT, 0) != hipSuccess) n = 1;
-                return n > 0 ? n : 1;
-            }();
-            int dev = 0, ncu = 0;
-            hipError_t e = hipGetDevice(&dev);
-            if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-            if (e != hipSuccess) return e;
-            const int64_t items = (int64_t)A * a.njobs * ((a.B + TB - 1) / TB);
-            int64_t g = (int64_t)ncu * occ;
-            if (items < g) g = items;
-            g = (g + 7) / 8 * 8;
-            if (bl) {
-                // batched: each slot's struct (with A set) is in bl->args
-                if constexpr (batch_length(L)) {
-                    const dim3 grid((unsigned)g, (unsigned)bl->nslots);
-                    const StridedArgs* arr = (const StridedArgs*)bl->args;
-                    if constexpr (STDP_OK) {
-                        if (stdp) {
-                            hipLaunchKernelGGL((k_sstage_b<TIN, TOUT, T, L, TB, NT, DIR, true>), grid, dim3(NT), 0, st, arr, bl->left);
-                            return hipGetLastError();
-                        }
-                    }
-                    hipLaunchKernelGGL((k_sstage_b<TIN, TOUT, T, L, TB, NT, DIR>), grid, dim3(NT), 0, st, arr, bl->left);
-                    return hipGetLastError();
-                }
-                return hipErrorNotSupported;
-            }
-            if constexpr (STDP_OK) {
-                if (stdp) {
-                    hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR, true>), dim3((unsigned)g), dim3(NT), 0, st, b);
-                    return hipGetLastError();
+            if constexpr (dualc_applies<TIN, T, L, TB, DIR>()) {
+                int nd = 0;
+                for (int j = 0; j < a.njobs; j++) nd += a.job[j].out2 != nullptr;
+                if (nd > 0 && dualc_enabled()) {
+                    hipError_t e = launch_sstage<TIN, TOUT, T, L, TB, DIR, 1>(b, A, nd, st, bl);
+                    if (e == hipSuccess && nd < a.njobs) e = launch_sstage<TIN, TOUT, T, L, TB, DIR, 2>(b, A, a.njobs - nd, st, bl);
+                    return e;
                 }
             }
-            hipLaunchKernelGGL((k_sstage<TIN, TOUT, T, L, TB, NT, DIR>), dim3((unsigned)g), dim3(NT), 0, st, b);
-            return hipGetLastError();
+            return launch_sstage<TIN, TOUT, T, L, TB, DIR, 0>(b, A, a.njobs, st, bl);
         }
     }
     if (bl) return hipErrorNotSupported;  // no batched form of the direct-load kernel

@@ -86,6 +86,9 @@ CASES = {
     # the y transform of the 1024^3 run (1536 points, 1025 retained k2) on
     # a thin x-z cross-section: the fused y pass + epilogue at that length
     "y1536_full_k2": dict(Nx=16, Ny=1024, Nz=16, nu=2e-3, C_At=0.5, force_eps=0.1),
+    # the x transform of the 512^3 run (768 points, 513 retained k1): the FP64
+    # inverse x pass runs both transforms of each dual job as one
+    "x768_full_k1": dict(Nx=512, Ny=16, Nz=16, nu=2e-3, C_At=0.5),
 }
 
 
