@@ -81,6 +81,7 @@ def lib():
         L.orc_fft_r2c.argtypes = [ctypes.c_int] * 3 + [D, D]
         L.orc_fft1d.argtypes = [ctypes.c_int, D, ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_prefault.argtypes = [P]
         _lib = L
     return _lib
 
@@ -168,6 +169,11 @@ class Oracle:
 
     def finish_init(self, isReLambda=False):
         self.L.orc_finish_init(self.h, int(isReLambda))
+
+    def prefault(self):
+        """Touch every page of the field arrays (values unchanged), so a
+        timed step does not pay first-touch page faults."""
+        self.L.orc_prefault(self.h)
 
     # --- stepping / forcing ---
     def step(self, n=1):

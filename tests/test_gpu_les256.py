@@ -35,6 +35,11 @@ def test_les_256_fp64_vs_oracle():
 FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "les256_k0_6_seed6.npz")
 
 
+def checkpoints(f):
+    """Steps at which the fixture holds E(K), norms and the |k| <= 8 modes."""
+    return sorted(int(k[len("low8_"):]) for k in f.files if k.startswith("low8_") and k[len("low8_"):].isdigit())
+
+
 def test_les_256_fp64_500_steps_vs_oracle_fixture():
     """The whole 500-step horizon of the 256^3 LES against the FP64 oracle's
     run of the same seeded field (tests/golden/make_les256_fixture.py wrote
@@ -58,7 +63,7 @@ def test_les_256_fp64_500_steps_vs_oracle_fixture():
     sc = g.scalars()
     worst = {"time": 0.0, "dt": rel(sc["dt"], f["dt"][0]), "ek": rel(sc["ek"], f["ek"][0]), "om": rel(sc["om"], f["om"][0]),
              "skew": 0.0, "ek_bins": 0.0, "norm": 0.0, "low8": 0.0, "low16": 0.0}
-    cps = sorted(int(k.split("_")[1]) for k in f.files if k.startswith("low8_"))
+    cps = checkpoints(f)
     for n in range(1, steps + 1):
         g.step(1)
         sc = g.scalars()
