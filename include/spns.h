@@ -150,7 +150,9 @@ int spns_energy_cascade(spns_ctx *ctx, double *ek, int32_t nbins, double *ek_tot
 /* The kernels this context's step runs (no reference counterpart;
  * diagnostics and tests): *zpass_kernel = 2 staged z pass, 1 block per
  * pencil, 0 wave per pencil; *fused_epilogue = 1 when the DNS step fuses its
- * forward y pass with the epilogue (one kernel), else 0.  Either may be NULL. */
+ * forward y pass with the epilogue (one kernel), 2 when that kernel also
+ * runs the next step's inverse y pass (the step's own inverse y pass then
+ * does nothing), else 0.  Either may be NULL. */
 int spns_pass_kernels(spns_ctx *ctx, int32_t *zpass_kernel, int32_t *fused_epilogue);
 /* HIT getters (src/HIT_LES.h:187-201, 277-291). */
 int spns_get_scalars(spns_ctx *ctx, spns_scalars *out);

@@ -392,10 +392,11 @@ class Solver:
 
     def pass_kernels(self):
         """(z-pass kernel: 2 staged, 1 block per pencil, 0 wave per pencil;
-        True if the forward y pass is fused with the epilogue)."""
+        fused epilogue: 0 separate kernels, 1 the forward y pass fused with the
+        epilogue, 2 that kernel also running the next step's inverse y pass)."""
         z, f = ctypes.c_int32(), ctypes.c_int32()
         self._chk(self.L.spns_pass_kernels(self.h, ctypes.byref(z), ctypes.byref(f)))
-        return z.value, bool(f.value)
+        return z.value, f.value
 
     def scalars(self):
         s = Scalars()

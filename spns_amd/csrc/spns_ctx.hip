@@ -216,6 +216,12 @@ struct spns_ctx {
     double* skew_partial = nullptr;
     unsigned long long* mv = nullptr;  // [2]
     unsigned long long* mv_save = nullptr;  // [2] (the QA energy check keeps the CFL maxima)
+    // x1v (device int): X1 slots 0-5 hold u_c and d/dy u_c of the current
+    // state, written by the previous step's k_yepi_inv; the step's inverse y
+    // pass then does nothing.  yinv: the step runs k_yepi_inv (grid_yinv).
+    int* x1v = nullptr;
+    bool yinv = false;
+    int grid_yinv = 0;
     double* ek = nullptr;
     double* forced = nullptr;
     double* ft = nullptr;
@@ -319,12 +325,14 @@ struct spns_ctx {
         alloc((void**)&skew_partial, sizeof(double) * 2 * grid_z);
         alloc((void**)&mv, sizeof(unsigned long long) * 2);
         alloc((void**)&mv_save, sizeof(unsigned long long) * 2);
+        alloc((void**)&x1v, sizeof(int));
         alloc((void**)&ek, sizeof(double) * nbins);
         alloc((void**)&ft, sizeof(double) * (nbins + 1));
         HIPCHK(hipMemsetAsync(skew_partial, 0, sizeof(double) * 2 * grid_z, st));
         HIPCHK(hipMemsetAsync(X1, 0, (size_t)n_x1 * x1_slot * tsz, st));
         HIPCHK(hipMemsetAsync(X2, 0, (size_t)9 * x2_slot * tsz, st));
         HIPCHK(hipMemsetAsync(mv, 0, sizeof(unsigned long long) * 2, st));
+        HIPCHK(hipMemsetAsync(x1v, 0, sizeof(int), st));
         for (int c = 0; c < 3; c++) {
             HIPCHK(hipMemsetAsync(uk[c], 0, sb, st));
             HIPCHK(hipMemsetAsync(R[c], 0, sb, st));
@@ -340,6 +348,15 @@ struct spns_ctx {
                 const int tb = strided_tile(My, prec);
                 const int64_t items = tb > 0 ? (int64_t)nk1 * ((nk3 + tb - 1) / tb) : 0;
                 if (items > 0) grid_yepi = yepi_grid(My, prec, nbins, nk2, items, grid_epi);
+                // the next step's inverse y pass inside the fused kernel
+                // (environment SPNS_YINV=0 keeps it a separate pass)
+                const char* yv = std::getenv("SPNS_YINV");
+                yinv = false;
+                grid_yinv = 0;
+                if (grid_yepi > 0 && !cfg.isNotCubic && !(yv && yv[0] == '0') && inv_y_args(false).padded_ok) {
+                    grid_yinv = yepi_grid(My, prec, nbins, nk2, items, grid_epi, true);
+                    yinv = grid_yinv > 0;
+                }
             }
         }
         DevScalars s{};
@@ -407,6 +424,7 @@ struct spns_ctx {
         for (int c = 0; c < 3; c++) { hipFree(uk[c]); hipFree(R[c]); hipFree(ukprev[c]); }
         hipFree(X1); hipFree(X2); hipFree(twx); hipFree(twy); hipFree(twz);
         hipFree(scal); hipFree(hist_partial); hipFree(om_partial); hipFree(ed_partial); hipFree(skew_partial); hipFree(mv); hipFree(mv_save);
+        hipFree(x1v);
         hipFree(ek); hipFree(forced); hipFree(ft); hipFree(invspec);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
@@ -436,10 +454,21 @@ struct spns_ctx {
         a.A = nk1;
         return a;
     }
+    // The step's inverse y pass (all six fields of the current state) does
+    // nothing while x1v is set; any other use rewrites X1 and clears it.
     void inv_y(bool velocity_only, cx<double>* const* src = nullptr)
     {
-        const StridedArgs a = inv_y_args(velocity_only, src);
+        StridedArgs a = inv_y_args(velocity_only, src);
+        if (yinv && !velocity_only && !src) a.skip = x1v;
+        else x1_stale();
         HIPCHK(launch_strided(My, prec, 1, prec == PREC_FP64, +1, a, a.A, st));
+    }
+
+    // X1 no longer holds the current state's inverse y pass (the state
+    // changed, or X1 was rewritten outside a step)
+    void x1_stale()
+    {
+        if (yinv) HIPCHK(launch_set_int(x1v, 0, st));
     }
 
     // x passes: rows = x (or i1), batch = (y outer, i3 contiguous)
@@ -595,6 +624,11 @@ struct spns_ctx {
     // of the shell partials the finalize sums
     int fwd_y_epilogue()
     {
+        if (yinv) {
+            const StridedArgs ia = inv_y_args(false);
+            HIPCHK(launch_yepi(My, prec, fwd_y_args(3), nk1, epi_args(EPI_STEP), grid_yinv, st, nullptr, &ia, x1v));
+            return grid_yinv;
+        }
         if (grid_yepi > 0) {
             HIPCHK(launch_yepi(My, prec, fwd_y_args(3), nk1, epi_args(EPI_STEP), grid_yepi, st));
             return grid_yepi;
@@ -942,6 +976,7 @@ struct spns_batch {
         if (!batch_length(c->Mx) || !batch_length(c->My) || !batch_length(c->Mz))
             throw Error(SPNS_EUNSUPPORTED, "no batched kernels for this grid's FFT lengths");
         HIPCHK(hipStreamSynchronize(st));        // no copy in flight reads this slot's pinned structs
+        c->x1_stale();                           // batched steps run the separate inverse y pass
         HIPCHK(hipStreamSynchronize(c->st));     // the context's own work is done
         SlotArgs& a = hs[s];
         a.invy = c->inv_y_args(false);
@@ -1131,6 +1166,7 @@ int spns_reconfigure(spns_ctx* ctx, const spns_config* cfg)
     API_BEGIN(ctx)
     if (!cfg) throw Error(SPNS_EINVAL, "null config");
     HIPCHK(hipSetDevice(ctx->cfg.device));
+    ctx->x1_stale();  // the state changes
     ctx->reconfigure(*cfg);
     API_END(ctx)
 }
@@ -1167,6 +1203,7 @@ const char* spns_last_error(const spns_ctx* ctx) { return ctx ? ctx->err.c_str()
 int spns_set_state(spns_ctx* ctx, const double* state)
 {
     API_BEGIN(ctx)
+    ctx->x1_stale();  // the state changes
     if (!state) throw Error(SPNS_EINVAL, "null state");
     std::vector<std::complex<double>> h(3 * (size_t)ctx->nmodes);
     std::memcpy(h.data(), state, sizeof(double) * 6 * ctx->nmodes);
@@ -1210,6 +1247,7 @@ int spns_init_from_physical(spns_ctx* ctx, const double* u, const double* v, con
                             int32_t nzf, double* ek_file, double* ens_file)
 {
     API_BEGIN(ctx)
+    ctx->x1_stale();  // the state changes
     if (!u || !v || !w || nxf < 2 || nyf < 2 || nzf < 2) throw Error(SPNS_EINVAL, "bad file field");
     // r2c of the file grid on the device (src/HIT_LES.cpp:557-583)
     spns_fft* f = nullptr;
@@ -1282,6 +1320,7 @@ int spns_init_from_physical(spns_ctx* ctx, const double* u, const double* v, con
 int spns_init_k41(spns_ctx* ctx, int32_t nranks)
 {
     API_BEGIN(ctx)
+    ctx->x1_stale();  // the state changes
     if (nranks < 1) throw Error(SPNS_EINVAL, "nranks must be >= 1");
     const int Mx = ctx->Mx, My = ctx->My, Mz2 = ctx->Mz / 2;
     std::vector<std::complex<double>> h(3 * (size_t)ctx->nmodes, 0.0);
@@ -1321,6 +1360,7 @@ int spns_init_k41(spns_ctx* ctx, int32_t nranks)
 int spns_init_random(spns_ctx* ctx, uint64_t seed, double k0, double urms)
 {
     API_BEGIN(ctx)
+    ctx->x1_stale();  // the state changes
     if (!(k0 > 0) || !(urms >= 0)) throw Error(SPNS_EINVAL, "k0 must be > 0 and urms >= 0");
     RandArgs r{};
     for (int c = 0; c < 3; c++) r.uk[c] = ctx->uk[c];
@@ -1401,6 +1441,7 @@ int spns_recalculate_reynolds_lambda(spns_ctx* ctx, double* relambda)
 int spns_forcing_energy_cascade(spns_ctx* ctx, const double* fek, int64_t last_input_rad, int32_t nullify)
 {
     API_BEGIN(ctx)
+    ctx->x1_stale();  // the state changes
     if (!fek || last_input_rad < 0) throw Error(SPNS_EINVAL, "bad forced spectrum");
     ctx->forcing_energy_cascade(fek, last_input_rad, nullify);
     API_END(ctx)
@@ -1409,6 +1450,7 @@ int spns_forcing_energy_cascade(spns_ctx* ctx, const double* fek, int64_t last_i
 int spns_complex_conjugate_correction(spns_ctx* ctx)
 {
     API_BEGIN(ctx)
+    ctx->x1_stale();  // the state changes
     HIPCHK(launch_ccc(ctx->shell_args(), ctx->st));
     ctx->reduce_state();
     API_END(ctx)
@@ -1455,7 +1497,7 @@ int spns_pass_kernels(spns_ctx* ctx, int32_t* zpass_kernel, int32_t* fused_epilo
         z.kernel = ctx->zkernel;
         *zpass_kernel = zpass_kernel_of(ctx->Mz, ctx->prec, z);
     }
-    if (fused_epilogue) *fused_epilogue = ctx->grid_yepi > 0 ? 1 : 0;
+    if (fused_epilogue) *fused_epilogue = ctx->yinv ? 2 : (ctx->grid_yepi > 0 ? 1 : 0);
     API_END(ctx)
 }
 

@@ -42,6 +42,7 @@ struct StridedArgs {
     int padded_ok;  // column tiles rounded up to the tile width stay inside every input buffer
     int64_t in_a_stride, in_row_stride, out_a_stride, out_row_stride;  // in complex elements
     const void* tw;
+    const int* skip;  // if set and *skip != 0 the launch does nothing (its outputs are already current)
 };
 
 enum { ZK_WAVE = 0, ZK_BLOCK = 1, ZK_STAGED = 2, ZK_STAGED_NOFUSE = 3 };
@@ -183,9 +184,13 @@ hipError_t launch_zpass(int L, int prec, int mode, const ZArgs& a, int grid, hip
 int zpass_grid(int L, int prec, const ZArgs& a, int64_t npen);  // < 0: no such length / HIP error
 int zpass_kernel(int L, int prec, const ZArgs& a);              // kernel launch_zpass picks
 hipError_t launch_zplain(int L, int prec, int dir, const ZPlainArgs& a, int grid, hipStream_t st);
-int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid);
+// inv: the variant with the next step's inverse y pass (k_yepi_inv), which
+// launch_yepi runs when ia (the inverse pass's arguments) is given; it sets
+// *x1v
+int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid, bool inv = false);
 hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st,
-                       const BatchLaunch* bl = nullptr);
+                       const BatchLaunch* bl = nullptr, const StridedArgs* ia = nullptr, int* x1v = nullptr);
+hipError_t launch_set_int(int* p, int v, hipStream_t st);
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st, const BatchLaunch* bl = nullptr);
 hipError_t launch_finalize(const FinArgs& a, hipStream_t st, const BatchLaunch* bl = nullptr);
 hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* out, hipStream_t st);

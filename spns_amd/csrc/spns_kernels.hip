@@ -52,6 +52,7 @@ template <typename TIN, typename TOUT, typename T, int L, int TB, int NT, int DI
 __global__ __launch_bounds__(NT) void k_strided(StridedArgs a)
 {
     __shared__ cx<T> lds[L * TB];
+    if (a.skip && *a.skip) return;
     // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
     // L2.  Remap the (padded, multiple-of-8) 1-D grid so every XCD owns a
     // contiguous run of tiles: neighbouring column tiles, which share cache
@@ -167,6 +168,7 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
     __shared__ __align__(16) cx<T> fbuf[L * NB];
     __shared__ __align__(16) cx<TIN> stg[SROWS * TB];
     __shared__ __align__(16) cx<T> twl[TWN];
+    if (a.skip && *a.skip) return;  // uniform: every thread reads the same flag
     const int t = threadIdx.x;
     const int hin = a.nin / 2, hout = a.nout / 2;
     const int ntiles = (a.B + TB - 1) / TB;
@@ -1586,14 +1588,27 @@ __global__ __launch_bounds__(NT) void k_epilogue_b(const EpiArgs* __restrict__ a
 // of the item goes through epi_mode.  The convective spectra never reach
 // HBM.  Shell histograms are per wave, as in k_epilogue.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int TB>
-__host__ __device__ constexpr size_t yepi_static_lds()
+// INV: after the epilogue of an item, the next step's inverse y pass of the
+// item's new state (the same (k1, k3 tile) columns): per component, the state
+// tile this block has just written is staged back through the cache
+// hierarchy (FP64, [2 (L/3) + 1][TB]) and transformed twice, plain and i k2,
+// into u_c and d/dy u_c (X1 slots c and 3 + c), with the same arithmetic as
+// the strided inverse pass at the next step; the kernel sets *x1v, which lets
+// that pass skip.  The item's X1 columns are the ones its forward transforms
+// read, so the writes touch nothing another item still needs.  The staging
+// tile grows to the FP64 state tile (at 512^3 FP32 the LDS is then 160,584
+// bytes with the 12 waves' histograms).
+// Staging tile of k_yepi (elements of cx<T>): the forward pass's input
+// tile, or with INV also a staged FP64 state tile [2 (L/3) + 1][TB].
+template <typename T, int L, int TB, bool INV>
+__host__ __device__ constexpr int yepi_stg_elems()
 {
-    return sizeof(cx<T>) * ((size_t)2 * L * TB + fft_twiddle_span<L, sizeof(T) == 8 ? 1 : 2>());
+    constexpr size_t a = (size_t)L * TB, b = INV ? ((size_t)(2 * (L / 3) + 1) * TB * sizeof(cx<double>) + sizeof(cx<T>) - 1) / sizeof(cx<T>) : 0;
+    return (int)(a > b ? a : b);
 }
 
-template <typename T, int L, int TB, int NT>
-__device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e)
+template <typename T, int L, int TB, int NT, bool INV = false>
+__device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e, const StridedArgs& ia, int* x1v)
 {
     constexpr int SPR = TB * (int)sizeof(cx<T>) / 16;  // 16-byte copy slots per staged row
     constexpr int EPS = 16 / (int)sizeof(cx<T>);
@@ -1602,13 +1617,15 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
     constexpr int TWN = fft_twiddle_span<L, TWM>();
     constexpr int NOUTMAX = 2 * (L / 3) + 1;  // the launcher checks nout against it
     constexpr int Q = (NOUTMAX * TB + NT - 1) / NT;
+    constexpr int STGN = yepi_stg_elems<T, L, TB, INV>();
     static_assert(L * TB * sizeof(cx<T>) >= NT * sizeof(double), "reduction scratch aliases fbuf");
     __shared__ __align__(16) cx<T> fbuf[L * TB];
-    __shared__ __align__(16) cx<T> stg[L * TB];
+    __shared__ __align__(16) cx<T> stg[STGN];
     __shared__ __align__(16) cx<T> twl[TWN];
     extern __shared__ double hist[];  // [NW][nbins]
     const int t = threadIdx.x;
     const int nb = e.nbins;
+    if (INV && blockIdx.x == 0 && t == 0) *x1v = 1;  // read by the next step's inverse y pass only
     for (int i = t; i < NW * nb; i += NT) hist[i] = 0.0;
     for (int i = t; i < TWN; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
     __syncthreads();
@@ -1637,6 +1654,19 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
     };
     auto load = [&](int b, int p) -> cx<T> { return stg[p * TB + b]; };
     auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * TB + b] = v; };
+    // INV: the FP64 state tile [nin][TB] of component c of item it
+    const cx<double>* stgd = (const cx<double>*)stg;
+    auto issue_inv = [&](int64_t it, int c) {
+        const int64_t ai = it / ntiles;
+        const int b0 = (int)(it - ai * ntiles) * TB;
+        const cx<double>* in = (const cx<double>*)ia.job[c].in + ai * ia.in_a_stride + b0;
+        const int ns = ia.nin * TB;
+        for (int m = wave; m * 64 < ns; m += NW) {
+            const int s = m * 64 + lane;
+            const int row = s / TB, col = s - row * TB;
+            if (s < ns) dma16(in + (int64_t)row * ia.in_row_stride + col, stg_base + (uint32_t)m * 1024u);
+        }
+    };
 
     int64_t it = lin;
     if (it < nitems) issue(it, 0);
@@ -1652,7 +1682,7 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
             constexpr int c = decltype(cc)::value;
             auto prefetch = [&]() {
                 if (c < 2) issue(it, c + 1);
-                else if (it + G < nitems) issue(it + G, 0);
+                else if (!INV && it + G < nitems) issue(it + G, 0);  // INV: the state parts come first
             };
             FftS<T, L, TB, NT, TB, -1, false, TWM>::template run<false>(fbuf, twl, load, store_lds, prefetch, t);
             lds_barrier();
@@ -1682,6 +1712,57 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
             lds_barrier();  // every wave's copies are in; fbuf is free again
         };
         static_for<3>(comp);
+        if constexpr (INV) {
+            // -------- the next step's inverse y pass of this item's new state
+            dma_wait();     // this wave's state stores are complete ...
+            lds_barrier();  // ... and every wave's
+            issue_inv(it, 0);
+            dma_wait();
+            lds_barrier();
+            const int ihin = ia.nin / 2;
+            // a runtime loop over the components, with the thread index through
+            // an opaque copy: per-thread addresses are recomputed per component
+            // rather than held across the item (register pressure)
+            for (int c = 0; c < 3; c++) {
+                const int tl = opaque(t);
+                auto next = [&]() {
+                    if (c < 2) issue_inv(it, c + 1);
+                    else if (it + G < nitems) issue(it + G, 0);
+                };
+                auto none = [] {};
+                int deriv = 0;  // 0 for the plain transform, 1 for the i k2 one
+                // the inverse pass's loader (its input precision is FP64: i k u
+                // formed in FP64, then rounded to T)
+                auto load_inv = [&](int b, int p) -> cx<T> {
+                    int i, k;
+                    if (p <= ihin) { i = p; k = p; }
+                    else if (p >= L - ihin) { i = ia.nin - (L - p); k = p - L; }
+                    else return cx<T>{(T)0, (T)0};
+                    const cx<double> v = stgd[i * TB + b];
+                    if (deriv) {
+                        const double kk = (double)k;
+                        return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
+                    }
+                    return cx<T>{(T)v.x, (T)v.y};
+                };
+                auto write_inv = [&](void* dst) {
+                    cx<T>* o = (cx<T>*)dst + ai * ia.out_a_stride + b0;
+                    for (int idx = tl; idx < L * TB; idx += NT) {
+                        const int i = idx / TB, b = idx - i * TB;
+                        if (b0 + b < ia.B) o[(int64_t)i * ia.out_row_stride + b] = fbuf[i * TB + b];
+                    }
+                };
+                FftS<T, L, TB, NT, TB, +1, false, TWM>::template run<false>(fbuf, twl, load_inv, store_lds, none, tl);
+                lds_barrier();
+                write_inv(ia.job[c].out);
+                deriv = 1;
+                FftS<T, L, TB, NT, TB, +1, false, TWM>::template run<true>(fbuf, twl, load_inv, store_lds, next, tl);
+                lds_barrier();
+                dma_wait();  // the next component's (or item's) tile has landed
+                write_inv(ia.job[c].out2);
+                lds_barrier();
+            }
+        }
     }
     epi_flush<NT>(e, hist, om, (double*)fbuf);
 }
@@ -1689,7 +1770,13 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
 template <typename T, int L, int TB, int NT>
 __global__ __launch_bounds__(NT) void k_yepi(StridedArgs a, EpiArgs e)
 {
-    yepi_body<T, L, TB, NT>(a, e);
+    yepi_body<T, L, TB, NT>(a, e, a, nullptr);
+}
+
+template <typename T, int L, int TB, int NT>
+__global__ __launch_bounds__(NT) void k_yepi_inv(StridedArgs a, EpiArgs e, StridedArgs ia, int* x1v)
+{
+    yepi_body<T, L, TB, NT, true>(a, e, ia, x1v);
 }
 
 template <typename T, int L, int TB, int NT>
@@ -1697,7 +1784,7 @@ __global__ __launch_bounds__(NT) void k_yepi_b(const StridedArgs* __restrict__ a
                                                const int* __restrict__ left)
 {
     if (left[blockIdx.y] <= 0) return;
-    yepi_body<T, L, TB, NT>(arr[blockIdx.y], earr[blockIdx.y]);
+    yepi_body<T, L, TB, NT>(arr[blockIdx.y], earr[blockIdx.y], arr[blockIdx.y], nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -2483,23 +2570,38 @@ __host__ __device__ constexpr int yepi_nt()
     return L > 1152 ? sstage_nt<L, TB>() / 2 : sstage_nt<L, TB>();
 }
 
+// LDS check and occupancy of k_yepi (INV false) or k_yepi_inv (true)
+template <typename T, int L, int TB, int NT, bool INV, class K>
+static int yepi_blocks_per_cu(K kern, int nbins, int max_spill)
+{
+    const size_t shm = sizeof(double) * (NT / 64) * (size_t)nbins;
+    const size_t lds = sizeof(cx<T>) * ((size_t)L * TB + yepi_stg_elems<T, L, TB, INV>() +
+                                        fft_twiddle_span<L, sizeof(T) == 8 ? 1 : 2>());
+    if (lds + shm > 160 * 1024 - 512) return 0;
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)) != hipSuccess || (int)fa.localSizeBytes > max_spill) return 0;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, NT, shm) != hipSuccess || n < 1) return 0;
+    return n;
+}
+
 template <typename T, int L>
-static int yepi_grid_L(int nbins, int nout, int64_t items, int max_grid)
+static int yepi_grid_L(int nbins, int nout, int64_t items, int max_grid, bool inv)
 {
     constexpr int TB = strided_tb<T, L>();
     if constexpr (TB <= 0 || (TB * sizeof(cx<T>)) % 16 != 0) {
         return 0;
     } else {
+        if (nout > 2 * (L / 3) + 1) return 0;
         constexpr int NT = yepi_nt<L, TB>();
-        const size_t shm = sizeof(double) * (NT / 64) * (size_t)nbins;
-        if (yepi_static_lds<T, L, TB>() + shm > 160 * 1024 - 512 || nout > 2 * (L / 3) + 1) return 0;
         // a variant that spills registers (the 1024-thread tiles) is slower
-        // than the two separate kernels
-        hipFuncAttributes fa;
-        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_yepi<T, L, TB, NT>)) != hipSuccess || fa.localSizeBytes > 0)
-            return 0;
-        int n = 0, dev = 0, ncu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_yepi<T, L, TB, NT>, NT, shm) != hipSuccess || n < 1) return 0;
+        // than the two separate kernels; k_yepi_inv may keep a few bytes of
+        // loop-invariant values in scratch (FP32 at L = 768: 16 bytes,
+        // reloaded once per item)
+        const int n = inv ? yepi_blocks_per_cu<T, L, TB, NT, true>(k_yepi_inv<T, L, TB, NT>, nbins, 16)
+                          : yepi_blocks_per_cu<T, L, TB, NT, false>(k_yepi<T, L, TB, NT>, nbins, 0);
+        if (n < 1) return 0;
+        int dev = 0, ncu = 0;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
         int64_t g = (int64_t)ncu * n;
@@ -2510,28 +2612,32 @@ static int yepi_grid_L(int nbins, int nout, int64_t items, int max_grid)
     }
 }
 
-int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid)
+int yepi_grid(int L, int prec, int nbins, int nout, int64_t items, int max_grid, bool inv)
 {
-#define X(LL) \
-    if (L == LL) return prec == PREC_FP64 ? yepi_grid_L<double, LL>(nbins, nout, items, max_grid) : yepi_grid_L<float, LL>(nbins, nout, items, max_grid);
+#define X(LL)                                                                                   \
+    if (L == LL)                                                                                \
+        return prec == PREC_FP64 ? yepi_grid_L<double, LL>(nbins, nout, items, max_grid, inv) \
+                                 : yepi_grid_L<float, LL>(nbins, nout, items, max_grid, inv);
     SPNS_FOR_EACH_L(X)
 #undef X
     return 0;
 }
 
 template <typename T, int L>
-static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st, const BatchLaunch* bl)
+static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st, const BatchLaunch* bl,
+                                const StridedArgs* ia, int* x1v)
 {
     constexpr int TB = strided_tb<T, L>();
     if constexpr (TB <= 0 || (TB * sizeof(cx<T>)) % 16 != 0) {
         return hipErrorNotSupported;
     } else {
-        constexpr int NT = yepi_nt<L, TB>();
         StridedArgs b = a;
         b.A = A;
         if (!a.padded_ok || a.nin != L || a.njobs != 3 || a.nout > 2 * (L / 3) + 1 || grid < 8 || grid % 8) return hipErrorInvalidValue;
-        const size_t shm = sizeof(double) * (NT / 64) * (size_t)e.nbins;
         if (bl) {
+            constexpr int NT = yepi_nt<L, TB>();
+            const size_t shm = sizeof(double) * (NT / 64) * (size_t)e.nbins;
+            if (ia) return hipErrorNotSupported;
             if constexpr (batch_length(L)) {
                 hipLaunchKernelGGL((k_yepi_b<T, L, TB, NT>), dim3(grid, bl->nslots), dim3(NT), shm, st, (const StridedArgs*)bl->args,
                                    (const EpiArgs*)bl->args2, bl->left);
@@ -2539,18 +2645,48 @@ static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, i
             }
             return hipErrorNotSupported;
         }
+        if (ia) {
+            // the inverse pass's items must be the forward pass's: same
+            // outer extent and column tiles, all rows of the transform out
+            bool ok = x1v && ia->padded_ok && ia->njobs == 3 && ia->nout == L && ia->nin <= 2 * (L / 3) + 1 && ia->B == a.B;
+            for (int j = 0; ok && j < 3; j++) ok = ia->job[j].in && ia->job[j].out && ia->job[j].out2;
+            if (!ok) return hipErrorInvalidValue;
+            StridedArgs ib = *ia;
+            ib.A = A;
+            constexpr int NT = yepi_nt<L, TB>();
+            const size_t shm = sizeof(double) * (NT / 64) * (size_t)e.nbins;
+            hipLaunchKernelGGL((k_yepi_inv<T, L, TB, NT>), dim3(grid), dim3(NT), shm, st, b, e, ib, x1v);
+            return hipGetLastError();
+        }
+        constexpr int NT = yepi_nt<L, TB>();
+        const size_t shm = sizeof(double) * (NT / 64) * (size_t)e.nbins;
         hipLaunchKernelGGL((k_yepi<T, L, TB, NT>), dim3(grid), dim3(NT), shm, st, b, e);
         return hipGetLastError();
     }
 }
 
-hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st, const BatchLaunch* bl)
+hipError_t launch_yepi(int L, int prec, const StridedArgs& a, int A, const EpiArgs& e, int grid, hipStream_t st, const BatchLaunch* bl,
+                       const StridedArgs* ia, int* x1v)
 {
-#define X(LL) \
-    if (L == LL) return prec == PREC_FP64 ? launch_yepi_L<double, LL>(a, A, e, grid, st, bl) : launch_yepi_L<float, LL>(a, A, e, grid, st, bl);
+#define X(LL)                                                                                        \
+    if (L == LL)                                                                                     \
+        return prec == PREC_FP64 ? launch_yepi_L<double, LL>(a, A, e, grid, st, bl, ia, x1v) \
+                                 : launch_yepi_L<float, LL>(a, A, e, grid, st, bl, ia, x1v);
     SPNS_FOR_EACH_L(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+// Sets an int flag in device memory (stream-ordered, no memset node).
+static __global__ void k_set_int(int* p, int v)
+{
+    *p = v;
+}
+
+hipError_t launch_set_int(int* p, int v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_epilogue(int prec, const EpiArgs& a, int grid, hipStream_t st, const BatchLaunch* bl)
