@@ -50,10 +50,13 @@ def dims(N):
     return M, nk1, nk2, nk3
 
 
-def pass_bytes(N, prec, les=False, fused_epilogue=False):
+def pass_bytes(N, prec, les=False, fused_epilogue=0):
     """Algorithmic (compulsory) HBM bytes of each pass of one step.  With the
-    DNS forward y pass fused with the epilogue (one kernel, timed as fwd_y)
-    the convective spectra never reach HBM."""
+    DNS forward y pass fused with the epilogue (fused_epilogue >= 1: one
+    kernel, timed as fwd_y) the convective spectra never reach HBM; with the
+    next step's inverse y pass in that kernel too (2) the step's inverse y
+    pass moves nothing and the new state is not read back from HBM (its
+    outputs are written by the fused kernel)."""
     M, nk1, nk2, nk3 = dims(N)
     sc = 16 if prec == 0 else 8  # complex of the transform precision
     KN = nk1 * nk2 * nk3
@@ -71,6 +74,9 @@ def pass_bytes(N, prec, les=False, fused_epilogue=False):
     if fused_epilogue:
         b["fwd_y"] = nf * x1 * sc + 6 * KN * 16 + 6 * KN * 16
         b["epilogue"] = 0
+    if fused_epilogue == 2:
+        b["fwd_y"] += 6 * x1 * sc
+        b["inv_y"] = 0
     return b
 
 
@@ -123,8 +129,9 @@ def run_form(N, prec, steps, warmup, seed, prof_steps, les=False):
     sc = s.scalars()
     ek = s.spectrum()
     passes = s.profile_passes(prof_steps) if prof_steps > 0 else {}
+    fused = s.pass_kernels()[1]
     s.close()
-    return {"wall_s": t1 - t0, "event_ms": ev_ms, "scalars": sc, "ek": ek, "passes": passes}
+    return {"wall_s": t1 - t0, "event_ms": ev_ms, "scalars": sc, "ek": ek, "passes": passes, "fused": fused}
 
 
 def load_traffic(kernel, workload):
@@ -308,11 +315,11 @@ def main():
     sps = a.steps / r["wall_s"]
     prec_main = 1 if main_form == "fp32" else 0
 
-    # spns_profile_passes reports the fused forward y pass + epilogue kernel
-    # under fwd_y and 0 for the epilogue
+    # the step's kernels (spns_pass_kernels): 0 separate forward y pass and
+    # epilogue, 1 fused, 2 fused with the next step's inverse y pass;
+    # spns_profile_passes times the fused kernel under fwd_y
     def fused(rr):
-        p = rr["passes"]
-        return bool(p) and p.get("epilogue", 1.0) < 0.02 and p.get("fwd_y", 0.0) > 0.0
+        return rr["fused"]
 
     # dominant kernel roofline from per-pass HIP-event timings
     pb = pass_bytes(N, prec_main, les=a.les, fused_epilogue=fused(r))
@@ -336,7 +343,8 @@ def main():
             "grid_points_steps_per_s": sp * M ** 3,
             "hbm_pct_model": 100.0 * model_bytes(N, prec, les=a.les) * sp / PEAK_HBM,
             "bytes_per_step_algorithmic": sum(pass_bytes(N, prec, les=a.les, fused_epilogue=fused(rr)).values()),
-            "fwd_y_fused_with_epilogue": fused(rr),
+            "fwd_y_fused_with_epilogue": fused(rr) >= 1,
+            "next_inv_y_fused": fused(rr) == 2,
             "pass_ms": rr["passes"],
             "ek_total": rr["scalars"]["ek"],
             "epsilon": rr["scalars"]["nu"] * rr["scalars"]["om"],

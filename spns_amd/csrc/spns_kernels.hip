@@ -1408,7 +1408,7 @@ static __device__ __forceinline__ EpiScalars epi_scalars(const EpiArgs& a)
 
 static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScalars& q, int64_t m, int k1, int k2, int k3,
                                          bool step, const cx<double>* C, const cx<double>* S, double* myh, double& om,
-                                         double& ed)
+                                         double& ed, cx<double>* Nout = nullptr)
 {
     const int ik2 = k1 * k1 + k2 * k2 + k3 * k3;
     const double rad2 = (double)ik2;
@@ -1486,6 +1486,7 @@ static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScala
             a.uk[c][m] = N[c];
             a.R[c][m] = R1[c];
             U[c] = N[c];
+            if (Nout) Nout[c] = N[c];
         }
     }
     // reductions on the (new) state
@@ -1589,15 +1590,17 @@ __global__ __launch_bounds__(NT) void k_epilogue_b(const EpiArgs* __restrict__ a
 // HBM.  Shell histograms are per wave, as in k_epilogue.
 // ---------------------------------------------------------------------------
 // INV: after the epilogue of an item, the next step's inverse y pass of the
-// item's new state (the same (k1, k3 tile) columns): per component, the state
-// tile this block has just written is staged back through the cache
-// hierarchy (FP64, [2 (L/3) + 1][TB]) and transformed twice, plain and i k2,
-// into u_c and d/dy u_c (X1 slots c and 3 + c), with the same arithmetic as
-// the strided inverse pass at the next step; the kernel sets *x1v, which lets
-// that pass skip.  The item's X1 columns are the ones its forward transforms
-// read, so the writes touch nothing another item still needs.  The staging
-// tile grows to the FP64 state tile (at 512^3 FP32 the LDS is then 160,584
-// bytes with the 12 waves' histograms).
+// item's new state (the same (k1, k3 tile) columns): component 0 goes into the
+// staging tile (FP64, [2 (L/3) + 1][TB]) as the epilogue computes each mode;
+// components 1 and 2 are staged back from the state this block has just
+// written (through the cache hierarchy), each copy issued during the previous
+// component's transforms (holding them in registers would spill in FP32).
+// Each component is transformed twice, plain and i k2, into u_c and d/dy u_c
+// (X1 slots c and 3 + c), with the same arithmetic as the strided inverse
+// pass at the next step; the kernel sets *x1v, which lets that pass skip.  The item's X1 columns are the ones its
+// forward transforms read, so the writes touch nothing another item still
+// needs.  The staging tile grows to the FP64 state tile (at 512^3 FP32 the
+// LDS is then 160,584 bytes with the 12 waves' histograms).
 // Staging tile of k_yepi (elements of cx<T>): the forward pass's input
 // tile, or with INV also a staged FP64 state tile [2 (L/3) + 1][TB].
 template <typename T, int L, int TB, bool INV>
@@ -1654,8 +1657,8 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
     };
     auto load = [&](int b, int p) -> cx<T> { return stg[p * TB + b]; };
     auto store_lds = [&](int b, int p, cx<T> v) { fbuf[p * TB + b] = v; };
-    // INV: the FP64 state tile [nin][TB] of component c of item it
-    const cx<double>* stgd = (const cx<double>*)stg;
+    // INV: the FP64 state tile [nin][TB] of the component being transformed
+    cx<double>* stgw = (cx<double>*)stg;
     auto issue_inv = [&](int64_t it, int c) {
         const int64_t ai = it / ntiles;
         const int b0 = (int)(it - ai * ntiles) * TB;
@@ -1704,7 +1707,15 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
                                 cx<double>{(double)keep[1][qq].x * inv_nf, (double)keep[1][qq].y * inv_nf},
                                 cx<double>{(double)v2.x * inv_nf, (double)v2.y * inv_nf}};  // Fourier_Normalization
                             const int64_t m = ((int64_t)i * e.nk1 + i1) * e.nk3p + k3;
-                            epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused);
+                            if constexpr (INV) {
+                                cx<double> nn[3];
+                                epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused, nn);
+                                stgw[i * TB + b] = nn[0];
+                            } else {
+                                epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused);
+                            }
+                        } else if constexpr (INV) {
+                            stgw[i * TB + b] = cx<double>{0.0, 0.0};  // padding columns: transformed, never written out
                         }
                     }
                 }
@@ -1714,9 +1725,9 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
         static_for<3>(comp);
         if constexpr (INV) {
             // -------- the next step's inverse y pass of this item's new state
-            dma_wait();     // this wave's state stores are complete ...
-            lds_barrier();  // ... and every wave's
-            issue_inv(it, 0);
+            // (component 0 is in the staging tile, published by the barrier
+            // that ended the epilogue; the copies of components 1 and 2 read
+            // the epilogue's stores, so every wave's stores complete first)
             dma_wait();
             lds_barrier();
             const int ihin = ia.nin / 2;
@@ -1738,7 +1749,7 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
                     if (p <= ihin) { i = p; k = p; }
                     else if (p >= L - ihin) { i = ia.nin - (L - p); k = p - L; }
                     else return cx<T>{(T)0, (T)0};
-                    const cx<double> v = stgd[i * TB + b];
+                    const cx<double> v = stgw[i * TB + b];
                     if (deriv) {
                         const double kk = (double)k;
                         return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
@@ -2595,10 +2606,10 @@ static int yepi_grid_L(int nbins, int nout, int64_t items, int max_grid, bool in
         if (nout > 2 * (L / 3) + 1) return 0;
         constexpr int NT = yepi_nt<L, TB>();
         // a variant that spills registers (the 1024-thread tiles) is slower
-        // than the two separate kernels; k_yepi_inv may keep a few bytes of
-        // loop-invariant values in scratch (FP32 at L = 768: 16 bytes,
-        // reloaded once per item)
-        const int n = inv ? yepi_blocks_per_cu<T, L, TB, NT, true>(k_yepi_inv<T, L, TB, NT>, nbins, 16)
+        // than the two separate kernels; k_yepi_inv may keep a few
+        // loop-invariant values in scratch (FP32 at L = 768: 20 bytes, three
+        // reloads per item outside the transforms)
+        const int n = inv ? yepi_blocks_per_cu<T, L, TB, NT, true>(k_yepi_inv<T, L, TB, NT>, nbins, 32)
                           : yepi_blocks_per_cu<T, L, TB, NT, false>(k_yepi<T, L, TB, NT>, nbins, 0);
         if (n < 1) return 0;
         int dev = 0, ncu = 0;
@@ -2648,7 +2659,7 @@ static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, i
         if (ia) {
             // the inverse pass's items must be the forward pass's: same
             // outer extent and column tiles, all rows of the transform out
-            bool ok = x1v && ia->padded_ok && ia->njobs == 3 && ia->nout == L && ia->nin <= 2 * (L / 3) + 1 && ia->B == a.B;
+            bool ok = x1v && ia->padded_ok && ia->njobs == 3 && ia->nout == L && ia->nin == a.nout && ia->B == a.B;
             for (int j = 0; ok && j < 3; j++) ok = ia->job[j].in && ia->job[j].out && ia->job[j].out2;
             if (!ok) return hipErrorInvalidValue;
             StridedArgs ib = *ia;
