@@ -32,7 +32,9 @@ throughput.
 Runs never share state: a run's result is the same alone and in the mix, bit
 for bit, batched or not (tests/test_gpu_serve.py, tests/test_gpu_batch.py).
 """
+import concurrent.futures as cf
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -214,6 +216,9 @@ class Server:
         self.used = 0.0
         self.reused = 0
         self._kept = {}
+        # a batch's first chunk captures its graph; an allocation on another
+        # thread meanwhile invalidates the capture, so the two exclude each other
+        self._capture_lock = threading.Lock()
 
     # Finished runs' contexts are kept per shape class and reused by later
     # runs of that class (spns_reconfigure): freeing device memory waits for
@@ -239,38 +244,57 @@ class Server:
             self._close_entry(e)
             self.used -= e["bytes"]
 
-    def _start(self, req, now):
+    def _claim(self, req, now):
+        """The bookkeeping half of admitting a run (the loop's thread): a kept
+        context of the run's class, or the bytes of a new one, on the budget."""
         args = req.solver_args(self.precision)
         # batched launches use the default (staged) z-pass kernels
         v = self.tuner.variant(args) if (self.tuner and not self.batched) else 0
         key = self._pool_key(args, v)
         nbytes = device_bytes(zpass_variant=v, **args)
         kept = self._kept.get(key)
-        pin = None
+        e = None
         if kept:
             e = kept.pop(0)
             if not kept:
                 del self._kept[key]
-            s, nbytes, pin = e["solver"], e["bytes"], e.get("pin")
+            nbytes = e["bytes"]
             self.reused += 1
-            try:
-                s.reconfigure(**args)
-            except SpnsError:
-                self._close_entry(e)
-                self.used -= nbytes
-                raise
         else:
-            s = Solver(zpass_variant=v, **args)
             self.used += nbytes
+        req.admitted = now
+        return {"req": req, "args": args, "v": v, "key": key, "bytes": nbytes, "entry": e}
+
+    def _init(self, c):
+        """The device half (any thread; it touches only this run's context):
+        create or reconfigure the context, seed the field, first antitransform.
+        On an error the context is closed and the error returned."""
+        req, e, pin, s = c["req"], c["entry"], None, None
+        with self._capture_lock:  # no allocation while a batch captures its graph
+            return self._init_locked(c, req, e, pin, s)
+
+    def _init_locked(self, c, req, e, pin, s):
         try:
+            if e is not None:
+                s, pin = e["solver"], e.get("pin")
+                s.reconfigure(**c["args"])
+            else:
+                s = Solver(zpass_variant=c["v"], **c["args"])
             s.init_random(seed=req.seed, k0=req.k0)
             s.finish_init(False)
-        except SpnsError:
-            self._close_entry({"solver": s, "pin": pin})
-            self.used -= nbytes
-            raise
-        req.admitted = now
-        return {"req": req, "solver": s, "done": 0, "waited": 0, "bytes": nbytes, "key": key, "pin": pin}
+        except SpnsError as err:
+            if s is not None:
+                self._close_entry({"solver": s, "pin": pin})
+            return err
+        return {"req": req, "solver": s, "done": 0, "waited": 0, "bytes": c["bytes"], "key": c["key"], "pin": pin}
+
+    def _start(self, req, now):
+        c = self._claim(req, now)
+        a = self._init(c)
+        if isinstance(a, SpnsError):
+            self.used -= c["bytes"]
+            raise a
+        return a
 
     def _release(self, a, keep):
         s = a.get("solver")
@@ -349,6 +373,11 @@ class Server:
         host = {"start": 0.0, "retire": 0.0, "attach": 0.0, "launch": 0.0, "poll": 0.0, "idle": 0.0}
         clock = time.perf_counter
         t0 = time.perf_counter()
+        # initialisations (context creation, seeded field, first antitransform:
+        # tens of ms each) run on a worker thread, so the loop keeps relaunching
+        # the batches meanwhile; each touches only its own context and stream
+        worker = cf.ThreadPoolExecutor(1)
+        inits = []  # (future, claim)
 
         def retire(g, slot, a, failed_err=None):
             nonlocal n_runs
@@ -385,15 +414,23 @@ class Server:
                         break
                     pending.pop(0)
                     tc = clock()
-                    try:
-                        a = self._start(r, now)
-                    except SpnsError as e:
-                        self._fail({"req": r}, e, now)
-                        done.append(r)
-                        continue
-                    finally:
-                        host["start"] += clock() - tc
+                    c = self._claim(r, now)
+                    inits.append((worker.submit(self._init, c), c))
+                    host["start"] += clock() - tc
                     n_runs += 1
+                    progressed = True
+                # initialised runs wait for a slot of their class's batch
+                for item in [x for x in inits if x[0].done()]:
+                    inits.remove(item)
+                    fut, c = item
+                    r = c["req"]
+                    a = fut.result()
+                    if isinstance(a, SpnsError):
+                        self.used -= c["bytes"]
+                        self._fail({"req": r}, a, time.perf_counter() - t0)
+                        done.append(r)
+                        n_runs -= 1
+                        continue
                     groups = classes.setdefault(class_key(r, self.precision), [])
                     g = next((g for g in groups if len(g["slots"]) + len(g["waiting"]) < self.nslots), None)
                     if g is None:
@@ -449,7 +486,12 @@ class Server:
                                 plan[slot] = min(self.chunk, a["req"].steps - a["done"])
                         if any(plan):
                             tc = clock()
-                            g["batch"].step(plan)
+                            if g.get("captured"):
+                                g["batch"].step(plan)
+                            else:
+                                with self._capture_lock:
+                                    g["batch"].step(plan)
+                                g["captured"] = True
                             host["launch"] += clock() - tc
                             g["busy"] = True
                             for slot, a in g["slots"].items():
@@ -466,6 +508,11 @@ class Server:
                         time.sleep(2e-4)
                     host["idle"] += clock() - tc
         finally:
+            worker.shutdown(wait=True)
+            for fut, c in inits:  # only after an exception in the loop
+                a = fut.result()
+                if not isinstance(a, SpnsError):
+                    self._release(a, keep=False)
             for groups in classes.values():
                 for g in groups:
                     g["batch"].close()
