@@ -258,3 +258,56 @@ def test_fused_inverse_y_matches_separate(monkeypatch, kw, precision, graph):
         assert np.array_equal(a, b)
     for k in ("time", "dt", "ek", "om", "skewness", "iter"):
         assert s[3][k] == f[3][k], k
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_inverse_y_across_state_changes(monkeypatch, precision):
+    """Every call between steps that changes the state or rewrites the work
+    buffers (conjugate correction, the QA energy check, invariant spectra, a
+    forcing spectrum, an antitransform, a reconfigured run, the K41 and
+    physical-field initialisations) leaves the fused next-step inverse y pass
+    bit-identical to the separate pass."""
+    kw = dict(Nx=32, nu=1e-3, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2)
+    out = {}
+    for mode in ("separate", "fused"):
+        if mode == "separate":
+            monkeypatch.setenv("SPNS_YINV", "0")
+        else:
+            monkeypatch.delenv("SPNS_YINV", raising=False)
+        g = _solver(precision=precision, gradient_invariants=True, keep_step_start=True, **kw)
+        assert g.pass_kernels()[1] == (2 if mode == "fused" else 1)
+        g.init_random(seed=13, k0=4.0)
+        g.finish_init(False)
+        rec = []
+        g.step(2)
+        g.ccc()
+        g.step(2)
+        g.check_energy(atol=1e-3)
+        g.step(1)
+        rec.append(g.invariant_spectra())
+        g.step(2)
+        ek = g.spectrum()
+        g.forcing_energy_cascade(ek, len(ek) - 1)
+        g.step(2)
+        g.antitransform()
+        g.step(2)
+        rec.append(g.get_state())
+        g.reconfigure(nu=2e-3)
+        g.init_random(seed=14, k0=4.0)
+        g.finish_init(False)
+        g.step(3)
+        rec.append(g.get_state())
+        g.init_k41(4)
+        g.finish_init(False)
+        g.step(2)
+        rec.append(g.get_state())
+        u = g.velocity_physical()
+        g.init_from_physical(u[0], u[1], u[2])
+        g.finish_init(False)
+        g.step(2)
+        rec.append(g.get_state())
+        rec.append(g.get_rhist())
+        out[mode] = rec
+        g.close()
+    for a, b in zip(out["separate"], out["fused"]):
+        assert np.array_equal(a, b)
