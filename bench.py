@@ -56,7 +56,8 @@ def pass_bytes(N, prec, les=False, fused_epilogue=0):
     kernel, timed as fwd_y) the convective spectra never reach HBM; with the
     next step's inverse y pass in that kernel too (2) the step's inverse y
     pass moves nothing and the new state is not read back from HBM (its
-    outputs are written by the fused kernel)."""
+    outputs are written by the fused kernel).  The state is FP64 (16 bytes a
+    mode) and the R history is held in the transform type (sc)."""
     M, nk1, nk2, nk3 = dims(N)
     sc = 16 if prec == 0 else 8  # complex of the transform precision
     KN = nk1 * nk2 * nk3
@@ -69,10 +70,10 @@ def pass_bytes(N, prec, les=False, fused_epilogue=0):
         "zpass": 9 * x2 * sc + nf * x2 * sc,
         "fwd_x": nf * x2 * sc + nf * x1 * sc,
         "fwd_y": nf * x1 * sc + nf * KN * sc,
-        "epilogue": nf * KN * sc + 6 * KN * 16 + 6 * KN * 16,
+        "epilogue": nf * KN * sc + 6 * KN * 16 + 6 * KN * sc,
     }
     if fused_epilogue:
-        b["fwd_y"] = nf * x1 * sc + 6 * KN * 16 + 6 * KN * 16
+        b["fwd_y"] = nf * x1 * sc + 6 * KN * 16 + 6 * KN * sc
         b["epilogue"] = 0
     if fused_epilogue == 2:
         b["fwd_y"] += 6 * x1 * sc
@@ -81,13 +82,14 @@ def pass_bytes(N, prec, les=False, fused_epilogue=0):
 
 
 def model_bytes(N, prec, les=False):
-    """Compulsory-byte model of a step: n_fft * 2 * s_t * M^3 + 12 * 16 * K_N
-    (one write + read of an M^3 field per 3-D transform, the FP64 state and
-    history read and written once)."""
+    """Compulsory-byte model of a step: n_fft * 2 * s_t * M^3 +
+    6 * (16 + 2 * s_t) * K_N (one write + read of an M^3 field per 3-D
+    transform; the FP64 state and the R history, held in the transform type,
+    read and written once)."""
     M, nk1, nk2, nk3 = dims(N)
     n_fft = 21 if les else 15
     s_t = 8 if prec == 0 else 4
-    return n_fft * 2 * s_t * M ** 3 + 12 * 16 * nk1 * nk2 * nk3
+    return n_fft * 2 * s_t * M ** 3 + 6 * (16 + 2 * s_t) * nk1 * nk2 * nk3
 
 
 def sync_all(solver):

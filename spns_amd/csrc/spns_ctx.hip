@@ -198,7 +198,7 @@ struct spns_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // device memory
     cx<double>* uk[3] = {};
-    cx<double>* R[3] = {};
+    void* R[3] = {};  // R history, complex of the transform type (tsz bytes per mode)
     cx<double>* ukprev[3] = {};
     void* X1 = nullptr;
     void* X2 = nullptr;
@@ -296,7 +296,7 @@ struct spns_ctx {
         const size_t sb = sizeof(cx<double>) * (size_t)nmodes_p;
         for (int c = 0; c < 3; c++) {
             alloc((void**)&uk[c], sb);
-            alloc((void**)&R[c], sb);
+            alloc(&R[c], tsz * (size_t)nmodes_p);
             if (keep_prev) alloc((void**)&ukprev[c], sb);
         }
         x1_slot = (int64_t)My * nk1 * nk3p;
@@ -335,7 +335,7 @@ struct spns_ctx {
         HIPCHK(hipMemsetAsync(x1v, 0, sizeof(int), st));
         for (int c = 0; c < 3; c++) {
             HIPCHK(hipMemsetAsync(uk[c], 0, sb, st));
-            HIPCHK(hipMemsetAsync(R[c], 0, sb, st));
+            HIPCHK(hipMemsetAsync(R[c], 0, tsz * (size_t)nmodes_p, st));
             if (keep_prev) HIPCHK(hipMemsetAsync(ukprev[c], 0, sb, st));
         }
         // fused forward y pass + epilogue for DNS steps (environment
@@ -399,7 +399,7 @@ struct spns_ctx {
         const size_t sb = sizeof(cx<double>) * (size_t)nmodes_p;
         for (int c3 = 0; c3 < 3; c3++) {
             HIPCHK(hipMemsetAsync(uk[c3], 0, sb, st));
-            HIPCHK(hipMemsetAsync(R[c3], 0, sb, st));
+            HIPCHK(hipMemsetAsync(R[c3], 0, tsz * (size_t)nmodes_p, st));
             if (keep_prev) HIPCHK(hipMemsetAsync(ukprev[c3], 0, sb, st));
         }
         DevScalars s{};
@@ -810,7 +810,7 @@ struct spns_ctx {
     void reset_history()
     {
         for (int c = 0; c < 3; c++) {
-            HIPCHK(hipMemsetAsync(R[c], 0, sizeof(cx<double>) * nmodes_p, st));
+            HIPCHK(hipMemsetAsync(R[c], 0, tsz * (size_t)nmodes_p, st));
             if (keep_prev) HIPCHK(hipMemsetAsync(ukprev[c], 0, sizeof(cx<double>) * nmodes_p, st));
         }
         DevScalars s = read_scalars();
@@ -1189,7 +1189,9 @@ int64_t spns_device_bytes(const spns_config* c)
     const int64_t nk3p = (nk3 + 15) / 16 * 16;
     const int64_t tsz = c->precision == SPNS_PREC_FP32_SPLIT ? 8 : 16;
     const int64_t nfwd = c->C_Smag > 0 ? 9 : 3;
-    const int64_t state = 16 * nk2 * nk1 * nk3p * ((c->isSelfAdaptive || c->gradient_invariants || c->keep_step_start) ? 9 : 6);
+    // state (and the step-start copy) in FP64, R history in the transform type
+    const int64_t state = 16 * nk2 * nk1 * nk3p * ((c->isSelfAdaptive || c->gradient_invariants || c->keep_step_start) ? 6 : 3) +
+                          tsz * nk2 * nk1 * nk3p * 3;
     const int64_t x1 = std::max<int64_t>(6, nfwd) * My * nk1 * nk3p * tsz;
     const int64_t x2 = 9 * My * Mx * nk3p * tsz;
     return state + x1 + x2 + (64 << 20);  // + partial sums, twiddles, scalars
@@ -1238,8 +1240,22 @@ int spns_get_rhist(spns_ctx* ctx, double* rh)
 {
     API_BEGIN(ctx)
     if (!rh) throw Error(SPNS_EINVAL, "null output");
-    auto h = ctx->download_state(ctx->R);
-    std::memcpy(rh, h.data(), sizeof(double) * 6 * ctx->nmodes);
+    if (ctx->prec == PREC_FP64) {
+        auto h = ctx->download_state((cx<double>* const*)ctx->R);
+        std::memcpy(rh, h.data(), sizeof(double) * 6 * ctx->nmodes);
+    } else {
+        // FP32-split: the history is held in FP32; widened here
+        std::vector<cx<float>> h(3 * (size_t)ctx->nmodes);
+        for (int c = 0; c < 3; c++)
+            HIPCHK(hipMemcpy2DAsync(h.data() + c * ctx->nmodes, sizeof(cx<float>) * ctx->nk3, ctx->R[c],
+                                    sizeof(cx<float>) * ctx->nk3p, sizeof(cx<float>) * ctx->nk3, (size_t)ctx->nk2 * ctx->nk1,
+                                    hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        for (size_t i = 0; i < h.size(); i++) {
+            rh[2 * i] = h[i].x;
+            rh[2 * i + 1] = h[i].y;
+        }
+    }
     API_END(ctx)
 }
 

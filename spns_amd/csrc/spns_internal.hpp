@@ -75,7 +75,7 @@ enum { EPI_STEP = 0, EPI_REDUCE = 1 };
 struct EpiArgs {
     const void* conv[9];  // 3 convective (+6 SGS for LES) transformed fields, compact box
     cx<double>* uk[3];
-    cx<double>* R[3];
+    void* R[3];  // R history: cx<T> of the kernel's transform type T (FP64 form: double; FP32-split: float)
     cx<double>* ukprev[3];
     int nk1, nk2, nk3, nk3p, nbins;
     int64_t nmodes;  // padded: nk2 * nk1 * nk3p

@@ -1406,6 +1406,10 @@ static __device__ __forceinline__ EpiScalars epi_scalars(const EpiArgs& a)
     return q;
 }
 
+// RT: the storage type of the R history (the transform type of the form:
+// the FP32-split form keeps R in FP32, whose convective part already carries
+// the FP32 transforms' rounding; R is combined in FP64 either way).
+template <typename RT>
 static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScalars& q, int64_t m, int k1, int k2, int k3,
                                          bool step, const cx<double>* C, const cx<double>* S, double* myh, double& om,
                                          double& ed, cx<double>* Nout = nullptr)
@@ -1415,8 +1419,13 @@ static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScala
     const int K = shell_of(ik2);
     cx<double> U[3], R0[3], R1[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) { U[c] = a.uk[c][m]; R0[c] = a.R[c][m]; }
+    for (int c = 0; c < 3; c++) U[c] = a.uk[c][m];
     if (step) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const cx<RT> r = ((const cx<RT>*)a.R[c])[m];
+            R0[c] = cx<double>{(double)r.x, (double)r.y};
+        }
         const double dt = q.dt, kap = q.kap, k05 = q.k05, alpha = q.alpha;
         const double ct = q.nu * rad2;
         const bool mean = !(rad2 > 0.0);
@@ -1484,7 +1493,7 @@ static __device__ __forceinline__ void epi_mode(const EpiArgs& a, const EpiScala
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             a.uk[c][m] = N[c];
-            a.R[c][m] = R1[c];
+            ((cx<RT>*)a.R[c])[m] = cx<RT>{(RT)R1[c].x, (RT)R1[c].y};
             U[c] = N[c];
             if (Nout) Nout[c] = N[c];
         }
@@ -1556,7 +1565,7 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& a)
                 }
             }
         }
-        epi_mode(a, q, m, k1, k2, k3, step, C, (step && a.les) ? S : nullptr, myh, om, ed);
+        epi_mode<T>(a, q, m, k1, k2, k3, step, C, (step && a.les) ? S : nullptr, myh, om, ed);
     }
     epi_flush<NT>(a, hist, om, red);
     if (a.ed_partial) {
@@ -1709,10 +1718,10 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
                             const int64_t m = ((int64_t)i * e.nk1 + i1) * e.nk3p + k3;
                             if constexpr (INV) {
                                 cx<double> nn[3];
-                                epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused, nn);
+                                epi_mode<T>(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused, nn);
                                 stgw[i * TB + b] = nn[0];
                             } else {
-                                epi_mode(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused);
+                                epi_mode<T>(e, q, m, k1, k2, k3, true, C, nullptr, myh, om, ed_unused);
                             }
                         } else if constexpr (INV) {
                             stgw[i * TB + b] = cx<double>{0.0, 0.0};  // padding columns: transformed, never written out
