@@ -113,7 +113,8 @@ int spns_get_state(spns_ctx *ctx, double *state);
  * complex double (the compact layout above restricted to those planes), for
  * checks on grids whose whole state does not fit the host comfortably. */
 int spns_get_state_planes(spns_ctx *ctx, int32_t i2_first, int32_t n_i2, double *out);
-/* R history (R of the last step), compact layout. */
+/* R history (R of the last step), compact layout, as complex double.  The
+ * FP32-split form holds it in FP32 on the device; it is widened here. */
 int spns_get_rhist(spns_ctx *ctx, double *rhist);
 /* HIT::Input_Real_Field (src/HIT_LES.cpp:542-599, 738-763): u,v,w are
  * [x][y][z] of nxf*nyf*nzf doubles; ek_file/ens_file receive
