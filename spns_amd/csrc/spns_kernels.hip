@@ -44,6 +44,21 @@ __device__ __forceinline__ int wrap_k(int i, int n)  // compact index -> wavenum
     return (i <= n / 2) ? i : i - n;
 }
 
+// i k v, formed in the input precision (src/HIT_LES.cpp:37) and rounded to
+// the transform type T.  The products are rounded before they enter a
+// butterfly: with contraction off, the compiler cannot fuse them into the
+// butterflies' additions in one loader and not in another, so every pass
+// that forms a derivative on load (the strided inverse passes, and the next
+// step's inverse y pass inside the fused forward y kernel) has the same
+// rounding.
+template <typename T, typename TIN>
+__device__ __forceinline__ cx<T> times_ik(TIN kk, cx<TIN> v)
+{
+#pragma clang fp contract(off)
+    const TIN re = -kk * v.y, im = kk * v.x;
+    return cx<T>{(T)re, (T)im};
+}
+
 // ---------------------------------------------------------------------------
 // Strided batched pass: transforms along a row dimension of [A][rows][B]
 // (B contiguous), TB consecutive columns per block.
@@ -78,11 +93,7 @@ __global__ __launch_bounds__(NT) void k_strided(StridedArgs a)
             else return cx<T>{(T)0, (T)0};
             if (bb >= a.B) return cx<T>{(T)0, (T)0};
             cx<TIN> v = in[(int64_t)i * a.in_row_stride + bb];
-            if (deriv) {
-                // i k u, formed in the input precision (src/HIT_LES.cpp:37)
-                const TIN kk = (TIN)k;
-                return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
-            }
+            if (deriv) return times_ik<T>((TIN)k, v);
             return cx<T>{(T)v.x, (T)v.y};
         };
         auto store = [&](int b, int p, cx<T> v) {
@@ -225,11 +236,7 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
             else if (p >= L - hin) { i = a.nin - (L - p); k = p - L; }
             else return cx<T>{(T)0, (T)0};
             const cx<TIN> v = stg[i * TB + (b & (TB - 1))];
-            if (JSEL == 1 ? b >= TB : deriv) {
-                // i k u, formed in the input precision (src/HIT_LES.cpp:37)
-                const TIN kk = (TIN)k;
-                return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
-            }
+            if (JSEL == 1 ? b >= TB : deriv) return times_ik<T>((TIN)k, v);
             return cx<T>{(T)v.x, (T)v.y};
         };
         auto load_std = [&](int b, int j, auto rc) -> cx<T> {
@@ -250,11 +257,7 @@ __device__ __forceinline__ void sstage_body(const StridedArgs& a)
                 }
                 const int i = lo ? p : p - (L - NIN);
                 const cx<TIN> v = stg[i * TB + (b & (TB - 1))];
-                if (JSEL == 1 ? b >= TB : deriv) {
-                    // i k u, formed in the input precision (src/HIT_LES.cpp:37)
-                    const TIN kk = (TIN)(lo ? p : p - L);
-                    return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
-                }
+                if (JSEL == 1 ? b >= TB : deriv) return times_ik<T>((TIN)(lo ? p : p - L), v);
                 return cx<T>{(T)v.x, (T)v.y};
             }
         };
@@ -1759,10 +1762,7 @@ __device__ __forceinline__ void yepi_body(const StridedArgs& a, const EpiArgs& e
                     else if (p >= L - ihin) { i = ia.nin - (L - p); k = p - L; }
                     else return cx<T>{(T)0, (T)0};
                     const cx<double> v = stgw[i * TB + b];
-                    if (deriv) {
-                        const double kk = (double)k;
-                        return cx<T>{(T)(-kk * v.y), (T)(kk * v.x)};
-                    }
+                    if (deriv) return times_ik<T>((double)k, v);
                     return cx<T>{(T)v.x, (T)v.y};
                 };
                 auto write_inv = [&](void* dst) {
@@ -2581,13 +2581,17 @@ hipError_t launch_inv_spectra(int prec, const InvSpecArgs& a, int grid, double* 
 // Fused forward y pass + step epilogue (DNS): grid for A x tiles items with
 // nbins shell bins, or 0 where the kernel does not apply (staging or LDS
 // does not fit, or more retained rows than the 3/2 padding leaves).
-// Block size of k_yepi: that of the staged strided pass, except for the
-// longest transforms, where half of it leaves LDS for the per-wave shell
-// histograms (the transforms then take two butterflies per thread).
+// Block size of k_yepi: that of the staged strided pass, except
+//   * for the longest transforms, where half of it leaves LDS for the
+//     per-wave shell histograms;
+//   * where that block has 1024 threads (L = 192 and 384 with 16-column
+//     tiles): 16 waves leave 128 VGPRs a thread, and the fused kernel
+//     spills there; 8 waves leave 256.
+// The transforms then take up to two butterflies per thread.
 template <int L, int TB>
 __host__ __device__ constexpr int yepi_nt()
 {
-    return L > 1152 ? sstage_nt<L, TB>() / 2 : sstage_nt<L, TB>();
+    return (L > 1152 || sstage_nt<L, TB>() >= 1024) ? sstage_nt<L, TB>() / 2 : sstage_nt<L, TB>();
 }
 
 // LDS check and occupancy of k_yepi (INV false) or k_yepi_inv (true)

@@ -191,6 +191,8 @@ def test_velocity_step_start(kw):
                                 dict(Nx=32, nu=1e-3, C_At=0.5, isSelfAdaptive=True),
                                 dict(Nx=16, Ny=16, Nz=512, nu=2e-3, C_At=0.5, omega=(0.0, 0.0, 3.0)),
                                 dict(Nx=32, Ny=32, Nz=16, Mx=48, My=48, Mz=24, Lx=2, Ly=2, Lz=1, isNotCubic=True, nu=1e-3, C_At=0.5),
+                                dict(Nx=16, Ny=128, Nz=16, nu=2e-3, C_At=0.5),
+                                dict(Nx=16, Ny=256, Nz=24, nu=2e-3, C_At=0.5, force_eps=0.1),
                                 dict(Nx=16, Ny=1024, Nz=16, nu=2e-3, C_At=0.5, force_eps=0.1)])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_fused_epilogue_matches_split(monkeypatch, kw, precision):
@@ -222,6 +224,8 @@ def test_fused_epilogue_matches_split(monkeypatch, kw, precision):
 YINV_CASES = [dict(Nx=32, nu=1e-3, C_At=0.5, force_eps=0.1, force_kmin=1, force_kmax=2),
               dict(Nx=32, nu=1e-3, C_At=0.5, isSelfAdaptive=True),
               dict(Nx=64, Ny=32, Nz=48, nu=2e-3, C_At=0.5, omega=(0.0, 0.0, 3.0)),
+              dict(Nx=16, Ny=128, Nz=16, nu=2e-3, C_At=0.5),
+              dict(Nx=16, Ny=256, Nz=24, nu=2e-3, C_At=0.5, force_eps=0.1),
               dict(Nx=16, Ny=512, Nz=16, nu=2e-3, C_At=0.5),
               dict(Nx=16, Ny=1024, Nz=16, nu=2e-3, C_At=0.5, force_eps=0.1)]
 
