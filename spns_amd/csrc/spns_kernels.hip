@@ -1543,16 +1543,11 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& a)
     const int nk1 = a.nk1, nk2 = a.nk2, nk3 = a.nk3, nk3p = a.nk3p;
     const double inv_nf = 1.0 / a.nf;
     const bool step = a.mode == EPI_STEP;
-    // one wave per (i2, i1) row of the compact box, lanes along i3: no
-    // per-mode index division, coalesced row reads
-    const int lane = threadIdx.x & 63;
-    const int nrow = nk1 * nk2;
-    for (int row = blockIdx.x * NW + (int)(threadIdx.x >> 6); row < nrow; row += gridDim.x * NW)
-    for (int i3 = lane; i3 < nk3; i3 += 64) {
+    auto mode = [&](int row, int i3) {
         const int i2 = row / nk1, i1 = row - i2 * nk1;
         const int64_t m = (int64_t)row * nk3p + i3;
         const int k1 = wrap_k(i1, nk1), k2 = wrap_k(i2, nk2), k3 = i3;
-        if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) continue;  // src/HIT_LES.cpp:867-871
+        if (a.notcubic && ((k1 % a.Lx) || (k2 % a.Ly) || (k3 % a.Lz))) return;  // src/HIT_LES.cpp:867-871
         cx<double> C[3], S[6];
         if (step) {
 #pragma unroll
@@ -1569,6 +1564,28 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& a)
             }
         }
         epi_mode<T>(a, q, m, k1, k2, k3, step, C, (step && a.les) ? S : nullptr, myh, om, ed);
+    };
+    // The first nfull = 64 floor(nk3 / 64) modes of each (i2, i1) row: one
+    // wave per row, lanes along i3 (no per-mode index division, coalesced
+    // row reads).  The rest of every row (nk3 mod 64 modes: 1 at the
+    // 2^n-cubed grids, where a wave per row would run its last pass with one
+    // lane) are spread over all threads of the grid, row-major.
+    const int lane = threadIdx.x & 63;
+    const int nrow = nk1 * nk2;
+    const int nfull = nk3 & ~63, rem = nk3 - nfull;
+    if (nfull > 0) {
+        for (int row = blockIdx.x * NW + (int)(threadIdx.x >> 6); row < nrow; row += gridDim.x * NW)
+            for (int i3 = lane; i3 < nfull; i3 += 64) mode(row, i3);
+    }
+    if (rem > 0) {
+        // row = floor(f / rem) through FP64: (f + 1/2) / rem is never within
+        // 1/(2 rem) of an integer, far above the product's rounding error
+        const double inv_rem = 1.0 / rem;
+        const int64_t ntail = (int64_t)nrow * rem;
+        for (int64_t f = (int64_t)blockIdx.x * NT + threadIdx.x; f < ntail; f += (int64_t)gridDim.x * NT) {
+            const int row = (int)(((double)f + 0.5) * inv_rem);
+            mode(row, nfull + (int)(f - (int64_t)row * rem));
+        }
     }
     epi_flush<NT>(a, hist, om, red);
     if (a.ed_partial) {
