@@ -914,17 +914,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
 // without divergence: 8.96 -> 8.67 ms at 512^3); in FP64 the column-major
 // stage-0 writes conflict in LDS and cost more than that saves.
 // ---------------------------------------------------------------------------
+// FP32 pair buffer layout: the swizzled 8-column form of FftS where its index
+// algebra applies and the compile-time stage-0 loader runs (L/R0 = 64 or 128:
+// 768 and 1536), else pitch 9.
+template <typename T, int L>
+__host__ __device__ constexpr bool zfuse_swz()
+{
+    return sizeof(T) == 4 && fft_swz_ok<L>() && (L / Plan<L>::r(0) == 64 || L / Plan<L>::r(0) == 128);
+}
+
 template <typename T, int L, int NT, int NKC, int TWM = (sizeof(T) == 8 ? 1 : 2)>
 __host__ __device__ constexpr size_t zfuse_lds_bytes()
 {
-    return sizeof(cx<T>) * ((size_t)L * 9 + (size_t)9 * zstage_spc<L>() + fft_twiddle_span<L, TWM>()) +
+    return sizeof(cx<T>) * ((size_t)L * (zfuse_swz<T, L>() ? 8 : 9) + (size_t)9 * zstage_spc<L>() + fft_twiddle_span<L, TWM>()) +
            3 * 9 * sizeof(void*);
 }
 
 template <typename T, int L, int NT, int NKC = 0, int TWM = (sizeof(T) == 8 ? 1 : 2), bool ZOPAQUE = (sizeof(T) == 4)>
 __device__ __forceinline__ void zfuse_body(const ZArgs& a)
 {
-    constexpr int LD = 9, NB = 8;
+    constexpr bool SWZ = zfuse_swz<T, L>();
+    constexpr int LD = SWZ ? 8 : 9, NB = 8;
     constexpr int SPC = zstage_spc<L>();
     constexpr int NW = NT / 64;
     constexpr int EPS = 16 / (int)sizeof(cx<T>);
@@ -942,6 +952,8 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
     double vmax_sa = 0.0, s2 = 0.0, s3 = 0.0;
     const double iAx = a.Ax, iAy = a.Ay, iAz = a.Az;
     const bool sa = a.sa;
+    // element (row, column) of the pair buffer
+    auto fidx = [](int row, int col) -> int { return SWZ ? fft_swz(row, col) : row * LD + col; };
 
     if (t < 9) srcp[t] = (const cx<T>*)a.f[t];
     for (int i = t; i < TWN; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
@@ -994,7 +1006,7 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
         auto load = [&](int b, int p) -> cx<T> {
             if (b >= 6) {
                 if (!havep) return cx<T>{(T)0, (T)0};
-                const cx<T> v = fbuf[p * LD + b];
+                const cx<T> v = fbuf[fidx(p, b)];
                 return cx<T>{v.x, -v.y};  // forward = conj(inverse(conj x))
             }
             if (!have) return cx<T>{(T)0, (T)0};
@@ -1029,7 +1041,7 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
             constexpr int p0 = LR0 * r;
             if (col >= 6) {
                 if (!havep) return cx<T>{(T)0, (T)0};
-                const cx<T> v = fbuf[(p0 + j) * LD + col];
+                const cx<T> v = fbuf[fidx(p0 + j, col)];
                 return cx<T>{v.x, -v.y};  // forward = conj(inverse(conj x))
             }
             if (!have) return cx<T>{(T)0, (T)0};
@@ -1063,13 +1075,18 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
             }
         };
         auto store = [&](int b, int p, cx<T> v) { fbuf[p * LD + b] = (b >= 6) ? cx<T>{v.x, -v.y} : v; };
+        auto store_swz = [&](int b, int, int li, cx<T> v) { fbuf[li] = (b >= 6) ? cx<T>{v.x, -v.y} : v; };
         auto prefetch = [&]() {
             if (have && nxt < a.npencil) issue(nxt);
         };
-        if constexpr (FAST)
+        if constexpr (SWZ) {
+            static_assert(FAST, "the swizzled layout is only built with the compile-time loader");
+            FftS<T, L, NB, NT, LD, +1, true, TWM, S0COL, true>::template run<false>(fbuf, twl, load_fast, store_swz, prefetch, tl);
+        } else if constexpr (FAST) {
             FftS<T, L, NB, NT, LD, +1, true, TWM, S0COL>::template run<false>(fbuf, twl, load_fast, store, prefetch, tl);
-        else
+        } else {
             FftS<T, L, NB, NT, LD, +1, true, TWM, S0COL>::template run<false>(fbuf, twl, load, store, prefetch, tl);
+        }
         lds_barrier();
         dma_wait();  // pencil n+1's sources (this wave's copies) have landed
         if (havep) {
@@ -1081,8 +1098,8 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
             cx<T>* const d2 = (cx<T>*)a.f[2] + offp;
             for (int idx = t; idx < 2 * nkv; idx += NT) {
                 const int q = idx >= nkv ? 1 : 0, k = idx - q * nkv;
-                const cx<T> Z = fbuf[k * LD + 6 + q];
-                const cx<T> Zc = fbuf[(k == 0 ? 0 : L - k) * LD + 6 + q];
+                const cx<T> Z = fbuf[fidx(k, 6 + q)];
+                const cx<T> Zc = fbuf[fidx(k == 0 ? 0 : L - k, 6 + q)];
                 const T h = (T)0.5;
                 (q ? d2 : d0)[k] = cx<T>{h * (Z.x + Zc.x), h * (Z.y - Zc.y)};
                 if (q == 0) d1[k] = cx<T>{h * (Z.y + Zc.y), h * (Zc.x - Z.x)};
@@ -1092,16 +1109,18 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
         if (have) {
             // -------- physical space of pencil n: products into columns 6-7
             for (int n = t; n < L; n += NT) {
-                cx<T>* row = &fbuf[n * LD];
+                // the row's 8 elements: 8n + c, XORed with (n >> 2) & 31 when swizzled
+                const int rb = SWZ ? 8 * n : n * LD, rh = SWZ ? ((n >> 2) & 31) : 0;
+                auto row = [&](int c) -> cx<T>& { return fbuf[SWZ ? ((rb + c) ^ rh) : rb + c]; };
                 T u, v, w, ux, uy, uz, vx, vy, vz, wx, wy, wz;
                 if constexpr (FAST) {
-                    u = row[0].x, uz = row[0].y, v = row[1].x, vz = row[1].y, w = row[2].x, wz = row[2].y;
-                    ux = row[3].x, uy = row[3].y, vx = row[4].x, vy = row[4].y, wx = row[5].x, wy = row[5].y;
+                    u = row(0).x, uz = row(0).y, v = row(1).x, vz = row(1).y, w = row(2).x, wz = row(2).y;
+                    ux = row(3).x, uy = row(3).y, vx = row(4).x, vy = row(4).y, wx = row(5).x, wy = row(5).y;
                 } else {
-                    u = row[0].x, v = row[0].y, w = row[1].x;
-                    ux = row[1].y, uy = row[2].x, uz = row[2].y;
-                    vx = row[3].x, vy = row[3].y, vz = row[4].x;
-                    wx = row[4].y, wy = row[5].x, wz = row[5].y;
+                    u = row(0).x, v = row(0).y, w = row(1).x;
+                    ux = row(1).y, uy = row(2).x, uz = row(2).y;
+                    vx = row(3).x, vy = row(3).y, vz = row(4).x;
+                    wx = row(4).y, wy = row(5).x, wz = row(5).y;
                 }
                 mu = fmax(mu, fabs(u));
                 mv = fmax(mv, fabs(v));
@@ -1110,8 +1129,8 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
                 s2 += (double)ux * (double)ux;
                 s3 += (double)ux * (double)ux * (double)ux;
                 // advective form (src/HIT_LES.cpp:118-120)
-                row[6] = cx<T>{u * ux + v * uy + w * uz, u * vx + v * vy + w * vz};
-                row[7] = cx<T>{u * wx + v * wy + w * wz, (T)0};
+                row(6) = cx<T>{u * ux + v * uy + w * uz, u * vx + v * vy + w * vz};
+                row(7) = cx<T>{u * wx + v * wy + w * wz, (T)0};
             }
             lds_barrier();
         }
