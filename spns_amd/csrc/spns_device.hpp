@@ -421,26 +421,39 @@ __host__ __device__ constexpr int fft_twiddle_span()
     return m;
 }
 
-// Swizzled 8-column layout of FftS (SWZ): element (row, b).
+// Swizzled 8-column layout of FftS (SWZ) for elements of ES bytes: rows are
+// grouped in 256-byte blocks of RPB = 256 / (8 ES) rows (4 in FP32, 2 in
+// FP64), and element (row, b) sits at
+// (8 row + b) ^ ((row / RPB) mod 8 RPB), inside its own block.
+template <int ES>
+__host__ __device__ constexpr int fft_swz_rpb()
+{
+    return 256 / (8 * ES);
+}
+
+template <int ES>
 __host__ __device__ __forceinline__ constexpr int fft_swz(int row, int b)
 {
-    return (8 * row + b) ^ ((row >> 2) & 31);
+    constexpr int RPB = fft_swz_rpb<ES>(), SH = RPB == 4 ? 2 : 1;
+    return (8 * row + b) ^ ((row >> SH) & (8 * RPB - 1));
 }
 
 // The swizzled stages index rows j + r*(L/R) (reads) and base + r*Ns
 // (writes) without per-access shifts: every L/R and every Ns after stage 0
-// a multiple of 4, and the first radix too (stage-0 rows R j + r).
-template <int L>
+// a multiple of RPB, and the first radix too (stage-0 rows R j + r).
+template <int L, int ES>
 __host__ __device__ constexpr bool fft_swz_ok()
 {
-    if (Plan<L>::r(0) % 4 != 0) return false;
+    constexpr int RPB = fft_swz_rpb<ES>();
+    if (RPB != 2 && RPB != 4) return false;
+    if (Plan<L>::r(0) % RPB != 0) return false;
     int ns = 1;
     for (int s = 0; s < Plan<L>::n; s++) {
         const int R = Plan<L>::r(s);
-        if (s > 0 && ((L / R) % 4 != 0 || ns % 4 != 0)) return false;
+        if (s > 0 && ((L / R) % RPB != 0 || ns % RPB != 0)) return false;
         ns *= R;
     }
-    return L % 32 == 0;
+    return L % RPB == 0;
 }
 
 // TWM, the twiddle source of a butterfly's R-1 factors W^r:
@@ -457,21 +470,23 @@ __host__ __device__ constexpr bool fft_swz_ok()
 // Stage 0 calls load(b, j, std::integral_constant<int, r>) for input
 // j + r*(L/R0) when the functor accepts that form, else load(b, j + r*(L/R0)).
 // SWZ: the LDS buffer holds 8 columns with element (row, b) at
-// fft_swz(row, b) = (8 row + b) ^ ((row >> 2) & 31) instead of row*LD + b.
-// The XOR stays inside aligned 32-element (256-byte in FP32) blocks, so the
-// buffer is exactly L*8 elements.  Every access pattern of the transform
-// (8 columns x 4 consecutive rows per 32 lanes, the stage-0 writes of rows
-// R j + r across lanes j) and a thread-per-row access of one column then
-// touch 32 distinct 8-byte bank slots, where the plain pitch-9 layout
+// fft_swz<sizeof(C)>(row, b) instead of row*LD + b.  The XOR stays inside
+// aligned 256-byte blocks, so the buffer is exactly L*8 elements.  The
+// transform's access patterns (8 columns x consecutive rows, the stage-0
+// writes of rows R j + r across lanes j) and a thread-per-row access of one
+// column then spread over all 64 banks, where the plain pitch-9 layout
 // conflicts two- to four-way.  The last stage passes the element's index to
-// store(b, p, index, v).  Needs fft_swz_ok<L>() (the index algebra below).
+// store(b, p, index, v).  Needs fft_swz_ok<L, sizeof(C)>() (the index algebra
+// below).
 template <typename T, int L, int NB, int NT, int LD, int DIR, bool SYNC0 = false, int TWM = (sizeof(T) == 8 ? 1 : 2),
           bool S0COL = false, bool SWZ = false>
 struct FftS {
     using C = cx<T>;
     static constexpr int NS = Plan<L>::n;
     static_assert(NS >= 2, "FftS needs at least two stages");
-    static_assert(!SWZ || (NB == 8 && LD == 8 && fft_swz_ok<L>()), "swizzled layout: 8 columns, pitch 8, fft_swz_ok");
+    static_assert(!SWZ || (NB == 8 && LD == 8 && fft_swz_ok<L, (int)sizeof(C)>()), "swizzled layout: 8 columns, pitch 8");
+    static constexpr int RPB = fft_swz_rpb<(int)sizeof(C)>();  // rows per 256-byte block (SWZ)
+    static constexpr int SH = RPB == 4 ? 2 : 1, SM = 8 * RPB - 1;
 
     template <int S, class Load, class Store, class Hook>
     __device__ static __forceinline__ void stage(C* __restrict__ lds, const C* __restrict__ twl, Load& load, Store& store,
@@ -503,9 +518,9 @@ struct FftS {
                         if constexpr (S == 0) {
                             v[q][r] = load(b, j + r * LR);
                         } else if constexpr (SWZ) {
-                            // row j + r LR: (row >> 2) = (j >> 2) + r LR/4, and 8 LR r is a
-                            // multiple of 32, outside the bits the XOR touches
-                            const int h = ((j >> 2) + ((r * (LR / 4)) & 31)) & 31;
+                            // row j + r LR: row / RPB = j / RPB + r LR / RPB, and 8 LR r is a
+                            // multiple of the block, outside the bits the XOR touches
+                            const int h = ((j >> SH) + ((r * (LR / RPB)) & SM)) & SM;
                             v[q][r] = lds[((8 * j + b) ^ h) + 8 * LR * r];
                         } else {
                             v[q][r] = lds[(j + r * LR) * LD + b];
@@ -560,14 +575,14 @@ struct FftS {
                 dft<R, DIR>(v[q]);
                 const int base = (j / Ns) * Ns * R + k;
                 // swizzled index of row base + r Ns (SWZ): for Ns a multiple of
-                // 4 as for the reads; at stage 0 (Ns = 1, base = R j with R a
-                // multiple of 4) the rows are base + r, (row >> 2) = base/4 + r/4
+                // RPB as for the reads; at stage 0 (Ns = 1, base = R j with R a
+                // multiple of RPB) the rows are base + r, row / RPB = base / RPB + r / RPB
                 auto widx = [&](int r) -> int {
-                    if constexpr (Ns % 4 == 0) {
-                        return ((8 * base + b) ^ (((base >> 2) + ((r * (Ns / 4)) & 31)) & 31)) + 8 * Ns * r;
+                    if constexpr (Ns % RPB == 0) {
+                        return ((8 * base + b) ^ (((base >> SH) + ((r * (Ns / RPB)) & SM)) & SM)) + 8 * Ns * r;
                     } else {
                         static_assert(!SWZ || Ns == 1, "swizzled stage rows");
-                        return 8 * base + ((8 * r + b) ^ (((base >> 2) + (r >> 2)) & 31));
+                        return 8 * base + ((8 * r + b) ^ (((base >> SH) + (r >> SH)) & SM));
                     }
                 };
 #pragma unroll

@@ -914,13 +914,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void
 // without divergence: 8.96 -> 8.67 ms at 512^3); in FP64 the column-major
 // stage-0 writes conflict in LDS and cost more than that saves.
 // ---------------------------------------------------------------------------
-// FP32 pair buffer layout: the swizzled 8-column form of FftS where its index
+// Pair buffer layout: the swizzled 8-column form of FftS where its index
 // algebra applies and the compile-time stage-0 loader runs (L/R0 = 64 or 128:
 // 768 and 1536), else pitch 9.
 template <typename T, int L>
 __host__ __device__ constexpr bool zfuse_swz()
 {
-    return sizeof(T) == 4 && fft_swz_ok<L>() && (L / Plan<L>::r(0) == 64 || L / Plan<L>::r(0) == 128);
+    return fft_swz_ok<L, (int)sizeof(cx<T>)>() && (L / Plan<L>::r(0) == 64 || L / Plan<L>::r(0) == 128);
 }
 
 template <typename T, int L, int NT, int NKC, int TWM = (sizeof(T) == 8 ? 1 : 2)>
@@ -953,7 +953,7 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
     const double iAx = a.Ax, iAy = a.Ay, iAz = a.Az;
     const bool sa = a.sa;
     // element (row, column) of the pair buffer
-    auto fidx = [](int row, int col) -> int { return SWZ ? fft_swz(row, col) : row * LD + col; };
+    auto fidx = [](int row, int col) -> int { return SWZ ? fft_swz<(int)sizeof(cx<T>)>(row, col) : row * LD + col; };
 
     if (t < 9) srcp[t] = (const cx<T>*)a.f[t];
     for (int i = t; i < TWN; i += NT) twl[i] = ((const cx<T>*)a.tw)[i];
@@ -1109,8 +1109,8 @@ __device__ __forceinline__ void zfuse_body(const ZArgs& a)
         if (have) {
             // -------- physical space of pencil n: products into columns 6-7
             for (int n = t; n < L; n += NT) {
-                // the row's 8 elements: 8n + c, XORed with (n >> 2) & 31 when swizzled
-                const int rb = SWZ ? 8 * n : n * LD, rh = SWZ ? ((n >> 2) & 31) : 0;
+                // the row's 8 elements: 8n + c, XORed with the row's swizzle
+                const int rb = SWZ ? 8 * n : n * LD, rh = SWZ ? fft_swz<(int)sizeof(cx<T>)>(n, 0) ^ (8 * n) : 0;
                 auto row = [&](int c) -> cx<T>& { return fbuf[SWZ ? ((rb + c) ^ rh) : rb + c]; };
                 T u, v, w, ux, uy, uz, vx, vy, vz, wx, wy, wz;
                 if constexpr (FAST) {
