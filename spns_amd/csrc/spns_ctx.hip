@@ -893,6 +893,7 @@ struct spns_batch {
     // asynchronous on the batch's stream) and device arrays
     struct SlotArgs {
         StridedArgs invy, invx, fwdx, fwdy;
+        StridedArgs invy2;  // the next step's inverse y pass inside the fused kernel (yinv classes)
         ZArgs z;
         EpiArgs epi;
         FinArgs fin;
@@ -904,7 +905,7 @@ struct spns_batch {
     // synchronous device read (which would wait behind other streams' work
     // sharing its hardware queue)
     DevScalars* hsc = nullptr;
-    StridedArgs *d_invy = nullptr, *d_invx = nullptr, *d_fwdx = nullptr, *d_fwdy = nullptr;
+    StridedArgs *d_invy = nullptr, *d_invx = nullptr, *d_fwdx = nullptr, *d_fwdy = nullptr, *d_invy2 = nullptr;
     ZArgs* d_z = nullptr;
     EpiArgs* d_epi = nullptr;
     FinArgs* d_fin = nullptr;
@@ -932,6 +933,7 @@ struct spns_batch {
             if (e != hipSuccess) { *p = nullptr; throw Error(SPNS_ENOMEM, "batch argument arrays: " + std::string(hipGetErrorString(e))); }
         };
         al((void**)&d_invy, sizeof(StridedArgs) * n); al((void**)&d_invx, sizeof(StridedArgs) * n);
+        al((void**)&d_invy2, sizeof(StridedArgs) * n);
         al((void**)&d_fwdx, sizeof(StridedArgs) * n); al((void**)&d_fwdy, sizeof(StridedArgs) * n);
         al((void**)&d_z, sizeof(ZArgs) * n); al((void**)&d_epi, sizeof(EpiArgs) * n);
         al((void**)&d_fin, sizeof(FinArgs) * n); al((void**)&d_clr, sizeof(ClearArgs) * n);
@@ -945,7 +947,7 @@ struct spns_batch {
     {
         if (st) hipStreamSynchronize(st);
         if (graph) hipGraphExecDestroy(graph);
-        hipFree(d_invy); hipFree(d_invx); hipFree(d_fwdx); hipFree(d_fwdy); hipFree(d_z); hipFree(d_epi);
+        hipFree(d_invy); hipFree(d_invx); hipFree(d_fwdx); hipFree(d_fwdy); hipFree(d_invy2); hipFree(d_z); hipFree(d_epi);
         hipFree(d_fin); hipFree(d_clr); hipFree(d_left);
         if (hs) hipHostFree(hs);
         if (hsc) hipHostFree(hsc);
@@ -961,7 +963,7 @@ struct spns_batch {
         return a->Nx == b->Nx && a->Ny == b->Ny && a->Nz == b->Nz && a->Mx == b->Mx && a->My == b->My && a->Mz == b->Mz &&
                a->prec == b->prec && a->les == b->les && a->nfwd == b->nfwd && a->zkernel == b->zkernel &&
                a->grid_z == b->grid_z && a->grid_epi == b->grid_epi && a->grid_yepi == b->grid_yepi && a->nbins == b->nbins &&
-               a->cfg.device == b->cfg.device;
+               a->yinv == b->yinv && a->grid_yinv == b->grid_yinv && a->cfg.device == b->cfg.device;
     }
 
     void attach(int s, spns_ctx* c)
@@ -976,23 +978,33 @@ struct spns_batch {
         if (!batch_length(c->Mx) || !batch_length(c->My) || !batch_length(c->Mz))
             throw Error(SPNS_EUNSUPPORTED, "no batched kernels for this grid's FFT lengths");
         HIPCHK(hipStreamSynchronize(st));        // no copy in flight reads this slot's pinned structs
-        c->x1_stale();                           // batched steps run the separate inverse y pass
+        c->x1_stale();                           // the first batched step runs the inverse y pass
         HIPCHK(hipStreamSynchronize(c->st));     // the context's own work is done
         SlotArgs& a = hs[s];
         a.invy = c->inv_y_args(false);
+        a.invy2 = StridedArgs{};
+        if (c->yinv) {
+            // as in the single-run step: the fused kernel runs the next step's
+            // inverse y pass and sets the slot's flag, which the batched inverse
+            // y pass reads
+            a.invy.skip = c->x1v;
+            a.invy2 = c->inv_y_args(false);
+            a.invy2.setflag = c->x1v;
+        }
         a.invx = c->inv_x_args(false);
         a.z = c->zpass_args(nullptr);
         a.fwdx = c->fwd_x_args();
         a.fwdy = c->fwd_y_args(c->nfwd);
         a.epi = c->epi_args(EPI_STEP);
         a.fin = c->fin_args(FIN_REDUCE | FIN_SKEW | FIN_MAXVEL | FIN_DT | FIN_ADVANCE);
-        a.fin.nblk_epi = c->grid_yepi > 0 ? c->grid_yepi : c->grid_epi;
+        a.fin.nblk_epi = c->yinv ? c->grid_yinv : c->grid_yepi > 0 ? c->grid_yepi : c->grid_epi;
         a.clr = ClearArgs{c->mv, c->mv + 1};
         auto up = [&](void* d, const void* h, size_t bytes) { HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st)); };
         up(d_invy + s, &a.invy, sizeof(StridedArgs)); up(d_invx + s, &a.invx, sizeof(StridedArgs));
         up(d_z + s, &a.z, sizeof(ZArgs)); up(d_fwdx + s, &a.fwdx, sizeof(StridedArgs));
         up(d_fwdy + s, &a.fwdy, sizeof(StridedArgs)); up(d_epi + s, &a.epi, sizeof(EpiArgs));
         up(d_fin + s, &a.fin, sizeof(FinArgs)); up(d_clr + s, &a.clr, sizeof(ClearArgs));
+        up(d_invy2 + s, &a.invy2, sizeof(StridedArgs));
         HIPCHK(hipMemcpyAsync(hsc + s, c->scal, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
         HIPCHK(hipEventRecord(ev_sc, st));
         HIPCHK(hipEventRecord(ev, st));
@@ -1030,7 +1042,12 @@ struct spns_batch {
         HIPCHK(launch_zpass(c->Mz, p, 0, hs[slot_of(c)].z, c->grid_z, st, &b));
         b = bl(d_fwdx);
         HIPCHK(launch_strided(c->Mx, p, p == PREC_FP64, p == PREC_FP64, -1, hs[slot_of(c)].fwdx, c->My, st, &b));
-        if (c->grid_yepi > 0) {
+        if (c->yinv) {
+            b = bl(d_fwdy, d_epi);
+            b.args3 = d_invy2;
+            HIPCHK(launch_yepi(c->My, p, hs[slot_of(c)].fwdy, c->nk1, hs[slot_of(c)].epi, c->grid_yinv, st, &b,
+                               &hs[slot_of(c)].invy2));
+        } else if (c->grid_yepi > 0) {
             b = bl(d_fwdy, d_epi);
             HIPCHK(launch_yepi(c->My, p, hs[slot_of(c)].fwdy, c->nk1, hs[slot_of(c)].epi, c->grid_yepi, st, &b));
         } else {

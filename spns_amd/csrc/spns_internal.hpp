@@ -43,6 +43,7 @@ struct StridedArgs {
     int64_t in_a_stride, in_row_stride, out_a_stride, out_row_stride;  // in complex elements
     const void* tw;
     const int* skip;  // if set and *skip != 0 the launch does nothing (its outputs are already current)
+    int* setflag;     // batched k_yepi_inv: as the inverse pass's arguments, the slot's flag the kernel sets
 };
 
 enum { ZK_WAVE = 0, ZK_BLOCK = 1, ZK_STAGED = 2, ZK_STAGED_NOFUSE = 3 };
@@ -160,6 +161,7 @@ struct BatchLaunch {
     const void* args2;
     int* left;
     int nslots;
+    const void* args3 = nullptr;  // k_yepi_inv: the slots' inverse y pass structs
 };
 
 constexpr int kMaxBatchSlots = 64;

@@ -1835,6 +1835,17 @@ __global__ __launch_bounds__(NT) void k_yepi_inv(StridedArgs a, EpiArgs e, Strid
     yepi_body<T, L, TB, NT, true>(a, e, ia, x1v);
 }
 
+// batched k_yepi_inv: slot = blockIdx.y; each slot's flag is its inverse
+// pass struct's setflag
+template <typename T, int L, int TB, int NT>
+__global__ __launch_bounds__(NT) void k_yepi_inv_b(const StridedArgs* __restrict__ arr, const EpiArgs* __restrict__ earr,
+                                                   const StridedArgs* __restrict__ iarr, const int* __restrict__ left)
+{
+    if (left[blockIdx.y] <= 0) return;
+    const StridedArgs& ia = iarr[blockIdx.y];
+    yepi_body<T, L, TB, NT, true>(arr[blockIdx.y], earr[blockIdx.y], ia, ia.setflag);
+}
+
 template <typename T, int L, int TB, int NT>
 __global__ __launch_bounds__(NT) void k_yepi_b(const StridedArgs* __restrict__ arr, const EpiArgs* __restrict__ earr,
                                                const int* __restrict__ left)
@@ -2697,8 +2708,18 @@ static hipError_t launch_yepi_L(const StridedArgs& a, int A, const EpiArgs& e, i
         if (bl) {
             constexpr int NT = yepi_nt<L, TB>();
             const size_t shm = sizeof(double) * (NT / 64) * (size_t)e.nbins;
-            if (ia) return hipErrorNotSupported;
             if constexpr (batch_length(L)) {
+                if (ia) {
+                    // with the next step's inverse y pass: ia is a slot's struct (all
+                    // slots share the shape class), the slots' structs are in args3
+                    bool ok = bl->args3 && ia->setflag && ia->padded_ok && ia->njobs == 3 && ia->nout == L && ia->nin == a.nout &&
+                              ia->B == a.B;
+                    for (int j = 0; ok && j < 3; j++) ok = ia->job[j].in && ia->job[j].out && ia->job[j].out2;
+                    if (!ok) return hipErrorInvalidValue;
+                    hipLaunchKernelGGL((k_yepi_inv_b<T, L, TB, NT>), dim3(grid, bl->nslots), dim3(NT), shm, st,
+                                       (const StridedArgs*)bl->args, (const EpiArgs*)bl->args2, (const StridedArgs*)bl->args3, bl->left);
+                    return hipGetLastError();
+                }
                 hipLaunchKernelGGL((k_yepi_b<T, L, TB, NT>), dim3(grid, bl->nslots), dim3(NT), shm, st, (const StridedArgs*)bl->args,
                                    (const EpiArgs*)bl->args2, bl->left);
                 return hipGetLastError();

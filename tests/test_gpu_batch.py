@@ -75,6 +75,15 @@ def test_batch_matches_alone(name):
     b.synchronize()
     late_alone.step(4)
     _same(late, late_alone, name + " late slot")
+    # a run that leaves the batch keeps stepping alone: the batched fused
+    # kernel has already run its next inverse y pass (DNS classes), and the
+    # run's own step must take that up (slot 1 took 4 steps in the last chunk)
+    alone[0].step(4)
+    _same(batched[0], alone[0], name + " slot 1")
+    b.detach(1)
+    batched[0].step(3)
+    alone[0].step(3)
+    _same(batched[0], alone[0], name + " slot 1 after leaving the batch")
     b.close()
     for s in batched + alone + [late, late_alone]:
         s.close()
