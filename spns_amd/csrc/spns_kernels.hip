@@ -2171,17 +2171,19 @@ static hipError_t launch_sstage(const StridedArgs& b, int A, int njsel, hipStrea
     return hipGetLastError();
 }
 
-// The combined dual-job kernel (JSEL 1) for the FP64 inverse passes at
-// L >= 512, unless SPNS_DUALC=0; the single jobs of the same pass then take
-// a second launch.  At 512^3 it took FP64 inv_x from 8.00 to 7.59 ms and
-// inv_y from 3.81 to 3.60 ms.  In FP32 it took inv_x from 4.01 to 3.76 ms,
-// but the z pass that follows ran 0.3 ms longer at the same cycle count
-// (rocprofv3 GRBM_GUI_ACTIVE: a lower engine clock after the faster
-// memory-bound kernel), so the FP32 step gained nothing and keeps JSEL 0.
+// The combined dual-job kernel (JSEL 1) for the inverse passes at L >= 512,
+// unless SPNS_DUALC=0; the single jobs of the same pass then take a second
+// launch.  At 512^3 it took FP64 inv_x from 8.00 to 7.59 ms and inv_y from
+// 3.81 to 3.60 ms.  In FP32 it took inv_x from 4.01 to 3.76 ms, but with the
+// pitch-9 z pass the z pass that follows ran 0.3 ms longer at the same cycle
+// count (rocprofv3 GRBM_GUI_ACTIVE: a lower engine clock after the faster
+// memory-bound kernel) and the step gained nothing; with the swizzled z pass
+// the FP32 inv_x goes from 3.98 to 3.72 ms and the step to 16.76 ms on one
+// box (16.81-17.03 ms without, on others), so both forms use it.
 template <typename TIN, typename T, int L, int TB, int DIR>
 constexpr bool dualc_applies()
 {
-    return DIR > 0 && sizeof(T) == 8 && L >= 512 && sstage_fits<TIN, T, L, TB, DIR, 2>();
+    return DIR > 0 && L >= 512 && sstage_fits<TIN, T, L, TB, DIR, 2>();
 }
 
 static bool dualc_enabled()
