@@ -37,6 +37,9 @@ CLASSES = {
     # LES, FP64, 64^3 (96^3): different C_s and viscosities
     "les64_fp64": (dict(Nx=64, precision=0), [dict(nu=1e-4, C_Smag=0.17, C_At=0.5), dict(nu=1e-3, C_Smag=0.1, C_At=0.5),
                                               dict(nu=3e-4, C_Smag=0.2, C_At=0.5, omega=(0.0, 0.0, 1.0))]),
+    # DNS, FP64, 32^3: the fused kernel with the next step's inverse y pass in FP64
+    "dns32_fp64": (dict(Nx=32, precision=0), [dict(nu=1e-3, C_At=0.5, force_eps=0.1), dict(nu=2e-3, C_At=0.5),
+                                              dict(nu=5e-4, fixed_dt=1e-3, omega=(0.0, 0.0, 2.0))]),
     # the rectangular serving grid 96x64x64 (144x96x96), DNS FP32-split, forced
     "rect96_fp32": (dict(Nx=96, Ny=64, Nz=64, precision=1), [dict(nu=1e-3, C_At=0.5, force_eps=0.1),
                                                              dict(nu=4e-3, C_At=0.5)]),
